@@ -1,0 +1,168 @@
+/*
+ * thysea.h -- C-ABI of the MI355X-native Tsunami-HySEA time-stepping library
+ * (libthysea.so, hand-written HIP kernels for gfx950).
+ *
+ * This is the drop-in boundary for the reference's host/GPU split:
+ *   reference: extern "C" int shallowWater(...53 args...)
+ *              (src/GPU/ShallowWater.cu:182-190, declared src/ShallowWater.cxx:19-27)
+ *   reference: int comprobarSoporteCUDA()  (src/GPU/SoporteCUDA.cu:8)
+ * The monolithic call is split into plain-C entry points (no C++ or HIP types in the
+ * signatures, plain pointers and sizes).  Inputs are the reference's non-dimensional
+ * host arrays (src/Problema.cxx:327-473); the caller owns host buffers, the context
+ * owns device memory, the host (not this library) owns files.
+ *
+ * Return codes follow the reference's int convention (0 ok, 1 device memory, 2 host
+ * memory; src/GPU/ShallowWater.cu:275,285,319) extended with the codes below.
+ */
+#ifndef THYSEA_H
+#define THYSEA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum thy_status {
+    THY_OK = 0,
+    THY_ERR_DEVICE_MEMORY = 1, /* as shallowWater's return 1 */
+    THY_ERR_HOST_MEMORY = 2,   /* as shallowWater's return 2 */
+    THY_ERR_INVALID = 3,       /* bad argument or call order */
+    THY_ERR_NO_DEVICE = 4,     /* no usable gfx950 device */
+    THY_ERR_HIP = 5,           /* a HIP runtime or launch error */
+    THY_ERR_BUDGET = 6,        /* batch admission refused by the memory budget */
+    THY_ERR_NONFINITE = 7      /* a health check found NaN/Inf in the state */
+};
+
+typedef struct thy_ctx thy_ctx;
+
+/* Grid geometry, as tipoDatosSubmalla (src/Constantes.hxx:32-38) built by
+   src/Problema.cxx:431-448.  Row j is the j-th latitude of the input file. */
+typedef struct thy_grid {
+    int nx, ny;
+    const double *lon;    /* [nx] degrees */
+    const double *lat;    /* [ny] degrees */
+    const double *area;   /* [ny] areaYCosPhi.x */
+    const double *cosphi; /* [ny] areaYCosPhi.y */
+    const double *ancho;  /* [ny+1] anchoVolumenes (dlon) */
+    const double *alto;   /* [ny] altoVolumenes (dlat) */
+} thy_grid;
+
+/* Scheme parameters, non-dimensional as passed to shallowWater
+   (src/GPU/ShallowWater.cu:186-190). */
+typedef struct thy_params {
+    double borde_sup, borde_inf, borde_izq, borde_der; /* 1 open, -1 wall, 2 periodic */
+    int sponge_sup, sponge_inf, sponge_izq, sponge_der;
+    double cfl, mf0, vmax, eps_h, hpos, cvis, dif_at;
+    double hmin;       /* Hmin shift applied to H */
+    double sea_level;  /* SEA_LEVEL / H */
+    double L, H, Q, T; /* scales */
+    int precision;     /* 0: FP32 state and fluxes; 1: FP64 everywhere */
+    int track_arrival; /* 1: arrival times tracked (reference behaviour) */
+    int tile_x, tile_y;/* 0: autotuned/default tile shape */
+} thy_params;
+
+/* One Okada fault, fields as a parameter-file fault line (src/Problema.cxx:128-148). */
+typedef struct thy_fault {
+    double time, lon_c, lat_c, depth_c, length, width, strike, dip, rake, slip;
+} thy_fault;
+
+/* Per-run schedule for thy_run (mirrors the loop of src/GPU/ShallowWater.cu:374-453). */
+typedef void (*thy_snapshot_fn)(void *user, int index, double t_nondim, thy_ctx *ctx);
+typedef void (*thy_okada_fn)(void *user, int fault_index);
+
+typedef struct thy_run_config {
+    double t_end;           /* tiempo_tot (non-dimensional) */
+    double save_interval;   /* tiempoGuardarNetCDF (non-dimensional, <0: none) */
+    double series_interval; /* tiempoGuardarSeries (non-dimensional, <0: none) */
+    int n_faults;           /* faults sorted by time (src/Problema.cxx:97-126) */
+    const thy_fault *faults;
+    int crop_flag;          /* 0 none, 1 relative, 2 absolute (non-dimensional) */
+    double crop_value;
+    int n_gauges;           /* gauge cell indices (-1: outside the grid) */
+    const int *gauge_pos;
+    int64_t max_steps;      /* 0: unlimited */
+    int steps_per_sync;     /* steps per captured graph replay (0: default) */
+    thy_snapshot_fn on_snapshot; /* called at each NetCDF snapshot time */
+    thy_okada_fn on_okada;       /* called before each Okada application */
+    void *user;
+} thy_run_config;
+
+typedef struct thy_run_result {
+    int64_t steps;
+    int n_samples;          /* gauge samples recorded */
+    int n_snapshots;
+    double t_final;         /* non-dimensional */
+    double dt_initial;      /* non-dimensional */
+    double seconds;         /* wall time of the stepping window (hipEvent) */
+} thy_run_result;
+
+/* Output fields (src/GPU/netcdf_kernel.cu:6-126). */
+enum thy_field {
+    THY_ETA = 0, THY_UX = 1, THY_UY = 2, THY_DEFORMED_BATHY = 3,
+    THY_MAX_HEIGHT = 4, THY_ARRIVAL_TIMES = 5
+};
+
+/* Device probe.  Replaces comprobarSoporteCUDA (src/GPU/SoporteCUDA.cu:8-29):
+   0 usable gfx950 device, 1 no GPU, 2 no gfx950 GPU. */
+int thy_probe(void);
+
+/* Allocate the device state of one grid.  Replaces the allocation part of
+   shallowWater (src/GPU/ShallowWater.cu:241-303). */
+int thy_create(const thy_grid *grid, const thy_params *params, thy_ctx **out);
+void thy_destroy(thy_ctx *ctx);
+const char *thy_last_error(const thy_ctx *ctx);
+
+/* Upload the reference's state arrays: state1 = (h, H) pairs and state2 = (qx, qy)
+   pairs, interleaved as the double2 arrays datosVolumenesNivel_1/_2
+   (src/GPU/ShallowWater.cu:299-300). */
+int thy_upload_state(thy_ctx *ctx, const double *state1, const double *state2);
+/* Inverse of thy_upload_state: (h, H) and (qx, qy) as doubles. */
+int thy_download_state(thy_ctx *ctx, double *state1, double *state2);
+
+/* eta_max / eta_ini / arrival-time initialisation (src/GPU/ShallowWater.cu:305-309). */
+int thy_init_metrics(thy_ctx *ctx);
+/* eta_max and arrival update from the current state at time t
+   (src/GPU/Metrics.cu:23-42,57-77). */
+int thy_update_metrics(thy_ctx *ctx, double t);
+
+/* Okada standard source (src/GPU/Deformacion.cu:197-222): FP64 displacement,
+   crop, H -= U_Z, eta_ini += U_Z.  def_out (optional, host) receives U_Z/H. */
+int thy_apply_okada(thy_ctx *ctx, const thy_fault *fault, int crop_flag, double crop_value,
+                    double *def_out);
+
+/* Initial dt (src/GPU/Timestep.cu:11-33), non-dimensional. */
+int thy_initial_dt(thy_ctx *ctx, double *dt);
+
+/* One step with a given dt (src/GPU/Timestep.cu:70-120); returns the next raw dt
+   (global minimum of the per-cell dt). */
+int thy_step(thy_ctx *ctx, double dt, double *dt_next);
+
+/* Whole driver loop of shallowWater (src/GPU/ShallowWater.cu:374-453): initial
+   dt, fault application and dt truncation at fault times (src/GPU/Timestep.cu:35-68),
+   snapshots, gauge series, metrics, steps.  Steps run in captured graph replays with
+   device-resident t and dt; the host synchronises once per replay. */
+int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res);
+
+/* Output field conversion to float (src/GPU/netcdf_kernel.cu), [ny*nx] row-major. */
+int thy_get_field(thy_ctx *ctx, int field, float *out);
+
+/* Gauge series recorded by the last thy_run: times [n_samples] (seconds) and eta/u/v
+   [n_samples * n_gauges] (src/GPU/ShallowWater.cu:69-90).  Pass NULL to skip. */
+int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v,
+                   float *eta_min, float *eta_max, float *bathy);
+
+/* Compensated FP64 sum of h*area (mass), for drift checks. */
+int thy_mass(thy_ctx *ctx, double *mass);
+
+/* Device bytes held by one context of this grid and parameters (admission). */
+size_t thy_footprint(int nx, int ny, const thy_params *params);
+
+/* Wall time (ms) of the last kernel launch group, for benchmarking. */
+int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

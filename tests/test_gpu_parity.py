@@ -1,0 +1,115 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 oracle.
+
+Tolerances (north star): eta, qx, qy within relative L-inf 1e-4 of the peak |eta|
+after the same step count with the oracle's dt replayed; relative mass drift 1e-6
+with walls; max-height and gauge series matching.
+"""
+import numpy as np
+import pytest
+
+from conftest import make_problem, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+def _native():
+    from thysea import native
+    return native
+
+
+def _oracle(pr, **kw):
+    import oracle as O
+    return O.OracleSim(pr, **kw)
+
+
+def test_probe_gfx950():
+    assert _native().probe() == 0
+
+
+def test_lake_at_rest():
+    pr = make_problem(fault=False)
+    s = _native().Solver(pr)
+    dt = s.initial_dt()
+    h0, H0, qx0, qy0 = s.state()
+    for _ in range(20):
+        s.step(dt)
+    h, H, qx, qy = s.state()
+    assert np.max(np.abs(qx)) < 1e-10 and np.max(np.abs(qy)) < 1e-10
+    assert np.max(np.abs(h - h0)) < 1e-9
+
+
+def test_initial_dt_matches_oracle():
+    pr = make_problem()
+    s = _native().Solver(pr)
+    o = _oracle(pr)
+    assert abs(s.initial_dt() - o.initial_dt()) / o.initial_dt() < 1e-5
+
+
+def test_okada_matches_oracle():
+    pr = make_problem(nx=96, ny=80)
+    s = _native().Solver(pr)
+    o = _oracle(pr)
+    f = pr.faults[0]
+    dg = s.okada(f, want_def=True)
+    do = o.okada(f, want_def=True)
+    scale = np.max(np.abs(do))
+    assert scale > 0
+    assert rel_linf(dg, do, scale) < 1e-10
+
+
+@pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, -1, -1, -1), (1, -1, 2, 2)])
+def test_step_replay(borders):
+    """Same dt sequence on both sides (the oracle's), compare after N steps."""
+    pr = make_problem(nx=70, ny=52, borders=borders)
+    s = _native().Solver(pr)
+    o = _oracle(pr)
+    f = pr.faults[0]
+    s.okada(f)
+    o.okada(f)
+    dt = o.initial_dt()
+    for _ in range(40):
+        dnext = o.step(dt)
+        s.step(dt)
+        dt = dnext
+    hg, Hg, qxg, qyg = s.state()
+    ho, Ho, qxo, qyo = o.state()
+    eta_o = ho - Ho - pr.hmin
+    eta_g = hg - Hg - pr.hmin
+    peak = np.max(np.abs(eta_o[ho > pr.eps_h]))
+    assert rel_linf(eta_g, eta_o, peak) < 1e-4
+    assert rel_linf(qxg, qxo, peak) < 1e-4
+    assert rel_linf(qyg, qyo, peak) < 1e-4
+
+
+def test_mass_walls():
+    pr = make_problem(nx=64, ny=48, borders=(-1, -1, -1, -1))
+    s = _native().Solver(pr)
+    s.okada(pr.faults[0])
+    m0 = s.mass()
+    dt = s.initial_dt()
+    for _ in range(60):
+        dt = s.step(dt)
+    assert abs(s.mass() - m0) / m0 < 1e-6
+
+
+def test_full_run_small_matches_oracle():
+    pr = make_problem(nx=96, ny=80, t_sim=900.0)
+    from thysea import scenario as S
+    glon, glat = S.gauges_lattice(pr.lon, pr.lat, 3)
+    from thysea import problem as P
+    pr.gauge_pos = np.array([P.nearest_cell(pr.lon, pr.lat, a, b) for a, b in zip(glon, glat)], np.int32)
+    pr.series_interval = 30.0 / pr.T
+    s = _native().Solver(pr)
+    rg = s.run()
+    o = _oracle(pr)
+    ro = o.run()
+    assert abs(rg["steps"] - ro["steps"]) <= 1
+    mg = s.field("max_height")
+    mo = o.output("max_height")
+    wet = mo > -9000
+    peak = np.max(np.abs(mo[wet]))
+    assert np.array_equal(mg > -9000, wet)
+    assert rel_linf(mg[wet], mo[wet], peak) < 1e-3
+    n = min(len(rg["g_time"]), len(ro["g_time"]))
+    assert n > 10
+    assert rel_linf(rg["g_eta"][:n], ro["g_eta"][:n], peak) < 2e-3

@@ -1,0 +1,792 @@
+// thy_api.hip -- C-ABI of libthysea.so (declared in include/thysea.h).
+//
+// Host side of the MI355X time-stepping library: device memory layout, launch
+// dispatch, graph-captured step batches and the driver loop with the semantics of
+// the reference's shallowWater (src/GPU/ShallowWater.cu:182-500).
+//
+// Device layout (structure of arrays, row pitch a multiple of 64 elements):
+//   e[2], qx[2], qy[2]   ping-pong state (eta relative to sea level, discharges)
+//   D                    still-water depth (changes only at Okada time)
+//   emax, tarr, eini     metrics (tarr/eini only when arrival times are tracked)
+//   row[ny], fy[ny+1]    per-row / per-face-row geometry constants
+//   ctrl[3], dtmin[3]    device control block and dt slots
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/thysea.h"
+#include "thy_aux.hpp"
+
+using namespace thy;
+
+struct thy_ctx {
+    int nx = 0, ny = 0, pitch = 0;
+    thy_params prm{};
+    int prec = 0;       // 0 float, 1 double
+    size_t esz = 4;
+    std::vector<double> lon, lat, area, cosphi, ancho, alto;
+    void *e[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
+    void *D = nullptr, *emax = nullptr, *tarr = nullptr, *eini = nullptr;
+    void *row = nullptr, *fy = nullptr;
+    double *area_d = nullptr;
+    Ctrl *ctrl = nullptr;
+    unsigned int *dtmin = nullptr;
+    unsigned int *scratch_u = nullptr;  // [0]: initial dt, [1]: okada max
+    double *ftime = nullptr;
+    int nf_cap = 0;
+    int *gpos = nullptr;
+    float *gbuf = nullptr, *gtime = nullptr;
+    int ng = 0, gcap = 0;
+    double *okada_def = nullptr;
+    double *mass_partial = nullptr;
+    float *outbuf = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int slot = 0;
+    int cur = 0;                 // host mirror of ctrl.cur after a sync
+    int tx = 64, ty = 16;
+    hipGraphExec_t gexec = nullptr;
+    int gk = 0;
+    double last_ms = 0.0;
+    std::vector<float> g_time, g_eta, g_u, g_v;
+    int nsamples = 0;
+    std::string err;
+};
+
+namespace {
+
+constexpr int kMassBlocks = 1024;
+
+int fail(thy_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? THY_ERR_DEVICE_MEMORY : THY_ERR_HIP, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+struct TileShape { int tx, ty; };
+
+// tile shapes compiled into the library (the autotuner chooses among these)
+template <typename T> bool tile_supported(int tx, int ty);
+template <> bool tile_supported<float>(int tx, int ty) {
+    return (tx == 64 && ty == 16) || (tx == 64 && ty == 8) || (tx == 32 && ty == 16) || (tx == 128 && ty == 8) ||
+           (tx == 64 && ty == 12);
+}
+template <> bool tile_supported<double>(int tx, int ty) { return (tx == 32 && ty == 8) || (tx == 64 && ty == 4); }
+
+template <typename T>
+StepArgs<T> make_args(thy_ctx *c) {
+    StepArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    const thy_params &p = c->prm;
+    a.nx = c->nx; a.ny = c->ny; a.pitch = c->pitch;
+    a.periodic = (std::fabs(p.borde_izq - 2.0) < 2.220446049250313e-16 && std::fabs(p.borde_der - 2.0) < 2.220446049250313e-16) ? 1 : 0;
+    a.tiles_x = (c->nx + c->tx - 1) / c->tx;
+    a.tiles_y = (c->ny + c->ty - 1) / c->ty;
+    a.n_tiles = a.tiles_x * a.tiles_y;
+    a.q_per_xcd = (a.n_tiles + 7) / 8;
+    a.sp_sup = p.sponge_sup; a.sp_inf = p.sponge_inf; a.sp_izq = p.sponge_izq; a.sp_der = p.sponge_der;
+    a.track_arrival = p.track_arrival;
+    a.n_gauges = c->ng;
+    a.bsup = (T)p.borde_sup; a.binf = (T)p.borde_inf; a.bizq = (T)p.borde_izq; a.bder = (T)p.borde_der;
+    a.cfl = (T)p.cfl; a.mf0 = (T)p.mf0; a.vmax = (T)p.vmax; a.eps_h = (T)p.eps_h; a.hpos = (T)p.hpos;
+    a.cvis = (T)p.cvis; a.dif_at = (T)p.dif_at; a.sea_level = (T)p.sea_level;
+    a.Href = (T)p.H; a.Q_over_H = (T)(p.Q / p.H);
+    for (int b = 0; b < 2; b++) {
+        a.e[b] = (T *)c->e[b]; a.qx[b] = (T *)c->qx[b]; a.qy[b] = (T *)c->qy[b];
+    }
+    a.D = (const T *)c->D;
+    a.emax = (T *)c->emax;
+    a.tarr = (T *)c->tarr;
+    a.eini = (const T *)c->eini;
+    a.row = (const RowGeo<T> *)c->row;
+    a.fy = (const FaceGeoY<T> *)c->fy;
+    a.ctrl = c->ctrl;
+    a.dtmin = c->dtmin;
+    a.ftime = c->ftime;
+    a.gpos = c->gpos;
+    a.gbuf = c->gbuf;
+    a.gtime = c->gtime;
+    return a;
+}
+
+template <typename T>
+hipError_t launch_step_t(thy_ctx *c, int slot) {
+    StepArgs<T> a = make_args<T>(c);
+    dim3 grid(a.q_per_xcd * 8), block(256);
+#define THY_TILE(TX_, TY_)                                                                            \
+    if (c->tx == TX_ && c->ty == TY_) {                                                              \
+        hipLaunchKernelGGL((k_step<T, TX_, TY_>), grid, block, 0, c->stream, a, slot);               \
+        return hipGetLastError();                                                                    \
+    }
+    if constexpr (sizeof(T) == 4) {
+        THY_TILE(64, 16)
+        THY_TILE(64, 8)
+        THY_TILE(32, 16)
+        THY_TILE(128, 8)
+        THY_TILE(64, 12)
+    } else {
+        THY_TILE(32, 8)
+        THY_TILE(64, 4)
+    }
+#undef THY_TILE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step(thy_ctx *c) {
+    hipError_t e = c->prec ? launch_step_t<double>(c, c->slot) : launch_step_t<float>(c, c->slot);
+    c->slot = (c->slot + 1) % 3;
+    return e;
+}
+
+int grid_1d(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 16384);
+}
+
+template <typename T>
+void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeoY<T>> &fys) {
+    const int ny = c->ny;
+    rows.resize(ny);
+    fys.resize(ny + 1);
+    for (int j = 0; j < ny; j++) {
+        const double A = c->area[j], cs = c->cosphi[j], al = c->alto[j];
+        RowGeo<T> r;
+        r.A = (T)A;
+        r.invA = (T)(1.0 / A);
+        r.gx = (T)(1.0 / cs);
+        r.valx = (T)((2.0 * (al / cs)) / (A + A));
+        r.lenx = (T)al;
+        r.dxs = (T)c->ancho[j];
+        r.dys = (T)al;
+        r.pad = T(0);
+        rows[j] = r;
+    }
+    for (int g = 0; g <= ny; g++) {
+        const int r0 = std::min(std::max(g - 1, 0), ny - 1), r1 = std::min(std::max(g, 0), ny - 1);
+        FaceGeoY<T> f;
+        const double len = c->ancho[g];
+        f.len = (T)len;
+        f.cos0 = (T)c->cosphi[r0];
+        f.cos1 = (T)c->cosphi[r1];
+        f.A0 = (T)c->area[r0];
+        f.A1 = (T)c->area[r1];
+        f.valy = (T)((2.0 * len) / (c->area[r0] + c->area[r1]));
+        f.ic0 = (T)(1.0 / c->cosphi[r0]);
+        f.ic1 = (T)(1.0 / c->cosphi[r1]);
+        fys[g] = f;
+    }
+}
+
+int pick_tile(thy_ctx *c) {
+    int tx = c->prm.tile_x, ty = c->prm.tile_y;
+    if (tx == 0 || ty == 0) {
+        if (c->prec) { tx = 32; ty = 8; }
+        else { tx = 64; ty = 16; }
+    }
+    bool ok = c->prec ? tile_supported<double>(tx, ty) : tile_supported<float>(tx, ty);
+    if (!ok) return fail(c, THY_ERR_INVALID, "unsupported tile shape " + std::to_string(tx) + "x" + std::to_string(ty));
+    c->tx = tx;
+    c->ty = ty;
+    return THY_OK;
+}
+
+template <typename T>
+int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
+    const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
+    const double hmin = ctx->prm.hmin;
+    std::vector<T> e((size_t)P * ny, T(0)), D((size_t)P * ny, T(0)), qx((size_t)P * ny, T(0)), qy((size_t)P * ny, T(0));
+    for (int j = 0; j < ny; j++) {
+        for (int i = 0; i < nx; i++) {
+            const size_t c = (size_t)j * nx + i, o = (size_t)j * P + i;
+            const double h = s1[2 * c], Hs = s1[2 * c + 1];
+            const double Dd = Hs + hmin;             // unshifted depth
+            D[o] = (T)Dd;
+            e[o] = (h > 0.0) ? (T)(h - Dd) : -(T)Dd;  // dry: e = -D exactly, so h = 0
+            qx[o] = (T)s2[2 * c];
+            qy[o] = (T)s2[2 * c + 1];
+        }
+    }
+    const size_t b = (size_t)P * ny * sizeof(T);
+    HIPCHK(hipMemcpy(ctx->e[0], e.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->e[1], e.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->D, D.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->qx[0], qx.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->qy[0], qy.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->qx[1], qx.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->qy[1], qy.data(), b, hipMemcpyHostToDevice));
+    ctx->cur = 0;
+    return THY_OK;
+}
+
+template <typename T>
+int download_t(thy_ctx *ctx, double *s1, double *s2) {
+    const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
+    const size_t b = (size_t)P * ny * sizeof(T);
+    std::vector<T> e((size_t)P * ny), D((size_t)P * ny), qx((size_t)P * ny), qy((size_t)P * ny);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(e.data(), ctx->e[ctx->cur], b, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(D.data(), ctx->D, b, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qx.data(), ctx->qx[ctx->cur], b, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qy.data(), ctx->qy[ctx->cur], b, hipMemcpyDeviceToHost));
+    for (int j = 0; j < ny; j++) {
+        for (int i = 0; i < nx; i++) {
+            const size_t c = (size_t)j * nx + i, o = (size_t)j * P + i;
+            if (s1) {
+                s1[2 * c] = (double)e[o] + (double)D[o];
+                s1[2 * c + 1] = (double)D[o] - ctx->prm.hmin;
+            }
+            if (s2) {
+                s2[2 * c] = (double)qx[o];
+                s2[2 * c + 1] = (double)qy[o];
+            }
+        }
+    }
+    return THY_OK;
+}
+
+int read_ctrl(thy_ctx *ctx, Ctrl *c) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(c, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+    return THY_OK;
+}
+
+int write_ctrl(thy_ctx *ctx, const Ctrl &c) {
+    HIPCHK(hipMemcpy(&ctx->ctrl[ctx->slot], &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return THY_OK;
+}
+
+int reset_dt_slots(thy_ctx *ctx) {
+    unsigned int v[3] = {0x7f7fffffu, 0x7f7fffffu, 0x7f7fffffu};
+    HIPCHK(hipMemcpy(ctx->dtmin, v, sizeof(v), hipMemcpyHostToDevice));
+    return THY_OK;
+}
+
+Ctrl idle_ctrl(thy_ctx *ctx) {
+    Ctrl c;
+    memset(&c, 0, sizeof(c));
+    c.t = 0.0;
+    c.t_end = 1e300;
+    c.next_save = INFINITY;
+    c.next_series = INFINITY;
+    c.series_dt = 0.0;
+    c.T = ctx->prm.T;
+    c.cur = ctx->cur;
+    c.gcap = ctx->gcap;
+    return c;
+}
+
+template <typename T>
+int apply_okada_t(thy_ctx *ctx, const thy_fault *f, int crop_flag, double crop_value, double *def_out) {
+    const int64_t n = (int64_t)ctx->nx * ctx->ny;
+    OkadaFault of;
+    of.lon_c = f->lon_c; of.lat_c = f->lat_c; of.depth = f->depth_c; of.L = f->length; of.W = f->width;
+    of.strike = f->strike; of.dip = f->dip; of.rake = f->rake; of.slip = f->slip; of.H = ctx->prm.H;
+    of.lon0 = ctx->lon[0];
+    of.dlon = (ctx->lon[ctx->nx - 1] - ctx->lon[0]) / (ctx->nx - 1);
+    of.lat0 = ctx->lat[0];
+    of.dlat = (ctx->lat[ctx->ny - 1] - ctx->lat[0]) / (ctx->ny - 1);
+    HIPCHK(hipMemsetAsync(&ctx->scratch_u[1], 0, sizeof(unsigned int), ctx->stream));
+    hipLaunchKernelGGL(k_okada, dim3(grid_1d(n)), dim3(256), 0, ctx->stream, of, ctx->nx, ctx->ny, ctx->okada_def,
+                       &ctx->scratch_u[1]);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_okada_apply<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, ctx->okada_def, ctx->nx,
+                       ctx->ny, ctx->pitch, crop_flag, crop_value, &ctx->scratch_u[1], (T *)ctx->e[ctx->cur],
+                       (T *)nullptr, (T *)ctx->D, (T *)ctx->eini);
+    HIPCHK(hipGetLastError());
+    if (def_out) {
+        HIPCHK(hipMemcpyAsync(def_out, ctx->okada_def, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return THY_OK;
+}
+
+template <typename T>
+int init_metrics_t(thy_ctx *ctx) {
+    const int64_t n = (int64_t)ctx->nx * ctx->ny;
+    hipLaunchKernelGGL((k_init_metrics<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, (const T *)ctx->e[ctx->cur],
+                       (const T *)ctx->D, ctx->nx, ctx->ny, ctx->pitch, (T)ctx->prm.eps_h, (T *)ctx->emax,
+                       (T *)ctx->eini, (T *)ctx->tarr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return THY_OK;
+}
+
+template <typename T>
+int initial_dt_t(thy_ctx *ctx, double *dt) {
+    const int64_t n = (int64_t)ctx->nx * ctx->ny;
+    StepArgs<T> a = make_args<T>(ctx);
+    unsigned int big = 0x7f7fffffu;
+    HIPCHK(hipMemcpy(&ctx->scratch_u[0], &big, sizeof(big), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((k_initial_dt<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, a, ctx->cur, &ctx->scratch_u[0]);
+    HIPCHK(hipGetLastError());
+    unsigned int r;
+    HIPCHK(hipMemcpy(&r, &ctx->scratch_u[0], sizeof(r), hipMemcpyDeviceToHost));
+    float f;
+    memcpy(&f, &r, 4);
+    *dt = (double)f;
+    return THY_OK;
+}
+
+template <typename T>
+int get_field_t(thy_ctx *ctx, int field, float *out) {
+    const int64_t n = (int64_t)ctx->nx * ctx->ny;
+    if (field == THY_ARRIVAL_TIMES && !ctx->tarr) {
+        for (int64_t i = 0; i < n; i++) out[i] = -1.0f;
+        return THY_OK;
+    }
+    hipLaunchKernelGGL((k_output<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, field, (const T *)ctx->e[ctx->cur],
+                       (const T *)ctx->qx[ctx->cur], (const T *)ctx->qy[ctx->cur], (const T *)ctx->D,
+                       (const T *)ctx->emax, (const T *)ctx->tarr, ctx->nx, ctx->ny, ctx->pitch, (T)ctx->prm.eps_h,
+                       ctx->prm.H, ctx->prm.Q, ctx->prm.T, ctx->outbuf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->outbuf, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return THY_OK;
+}
+
+template <typename T>
+int update_metrics_t(thy_ctx *ctx, double t) {
+    const int64_t n = (int64_t)ctx->nx * ctx->ny;
+    hipLaunchKernelGGL((k_update_metrics<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream,
+                       (const T *)ctx->e[ctx->cur], (const T *)ctx->D, ctx->nx, ctx->ny, ctx->pitch,
+                       (T)ctx->prm.eps_h, (T)ctx->prm.dif_at, (T)t, (T *)ctx->emax, (const T *)ctx->eini,
+                       (T *)ctx->tarr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return THY_OK;
+}
+
+template <typename T>
+int mass_t(thy_ctx *ctx, double *mass) {
+    hipLaunchKernelGGL((k_mass<T>), dim3(kMassBlocks), dim3(256), 0, ctx->stream, (const T *)ctx->e[ctx->cur],
+                       (const T *)ctx->D, (const RowGeo<T> *)ctx->row, ctx->area_d, ctx->nx, ctx->ny, ctx->pitch,
+                       ctx->mass_partial);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(kMassBlocks);
+    HIPCHK(hipMemcpy(part.data(), ctx->mass_partial, kMassBlocks * sizeof(double), hipMemcpyDeviceToHost));
+    double s = 0.0, comp = 0.0;
+    for (double x : part) {
+        double t = s + x;
+        if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
+        else comp += (x - t) + s;
+        s = t;
+    }
+    *mass = s + comp;
+    return THY_OK;
+}
+
+// Capture K consecutive step launches (slots 0,1,2,...) into one graph.
+int build_graph(thy_ctx *ctx, int K) {
+    if (ctx->gexec && ctx->gk == K) return THY_OK;
+    if (ctx->gexec) {
+        (void)hipGraphExecDestroy(ctx->gexec);
+        ctx->gexec = nullptr;
+    }
+    hipGraph_t g;
+    int slot_saved = ctx->slot;
+    ctx->slot = 0;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; k++) {
+        hipError_t e = launch_step(ctx);
+        if (e != hipSuccess) {
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            return fail(ctx, THY_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    HIPCHK(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    ctx->gk = K;
+    ctx->slot = slot_saved;
+    return THY_OK;
+}
+
+// Advance until the device stop predicate holds; graph replays of K steps.
+int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
+    int r;
+    while (ctx->slot != 0) {  // align to the graph's slot sequence
+        hipError_t e = launch_step(ctx);
+        if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, hipGetErrorString(e));
+    }
+    if ((r = build_graph(ctx, K)) != THY_OK) return r;
+    double total = 0.0;
+    for (;;) {
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        ctx->slot = (ctx->slot + K) % 3;
+        Ctrl c;
+        if ((r = read_ctrl(ctx, &c)) != THY_OK) return r;
+        float f = 0.f;
+        (void)hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
+        total += f;
+        ctx->cur = c.cur;
+        // After an executed step dt_valid is 0; it stays 1 only when a launch of this
+        // replay found the stop predicate true (every later launch then idled).
+        if (c.dt_valid) {
+            *out = c;
+            if (ms) *ms = total;
+            return THY_OK;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int thy_probe(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 1;
+    for (int d = 0; d < n; d++) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, d) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) return 0;
+    }
+    return 2;
+}
+
+size_t thy_footprint(int nx, int ny, const thy_params *p) {
+    const size_t esz = (p && p->precision) ? 8 : 4;
+    const size_t pitch = ((size_t)nx + 63) / 64 * 64;
+    const size_t n = pitch * ny;
+    size_t fields = 7 + 1 + ((p && p->track_arrival) ? 2 : 0);  // e,qx,qy x2, D, emax, (tarr, eini)
+    size_t bytes = fields * n * esz;
+    bytes += (size_t)nx * ny * (8 + 4);                 // okada scratch (double) + output staging (float)
+    bytes += (size_t)(ny + 1) * 64 * 2;                 // geometry
+    return bytes;
+}
+
+int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
+    if (!g || !p || !out || g->nx < 2 || g->ny < 2) return THY_ERR_INVALID;
+    thy_ctx *ctx = new (std::nothrow) thy_ctx();
+    if (!ctx) return THY_ERR_HOST_MEMORY;
+    *out = ctx;
+    ctx->nx = g->nx;
+    ctx->ny = g->ny;
+    ctx->prm = *p;
+    ctx->prec = p->precision ? 1 : 0;
+    ctx->esz = ctx->prec ? 8 : 4;
+    ctx->pitch = (g->nx + 63) / 64 * 64;
+    ctx->lon.assign(g->lon, g->lon + g->nx);
+    ctx->lat.assign(g->lat, g->lat + g->ny);
+    ctx->area.assign(g->area, g->area + g->ny);
+    ctx->cosphi.assign(g->cosphi, g->cosphi + g->ny);
+    ctx->ancho.assign(g->ancho, g->ancho + g->ny + 1);
+    ctx->alto.assign(g->alto, g->alto + g->ny);
+    int r = pick_tile(ctx);
+    if (r != THY_OK) return r;
+    HIPCHK(hipSetDevice(0));
+    HIPCHK(hipStreamCreate(&ctx->stream));
+    HIPCHK(hipEventCreate(&ctx->ev0));
+    HIPCHK(hipEventCreate(&ctx->ev1));
+    const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(hipMalloc(&ctx->e[k], b));
+        HIPCHK(hipMalloc(&ctx->qx[k], b));
+        HIPCHK(hipMalloc(&ctx->qy[k], b));
+    }
+    HIPCHK(hipMalloc(&ctx->D, b));
+    HIPCHK(hipMalloc(&ctx->emax, b));
+    if (p->track_arrival) {
+        HIPCHK(hipMalloc(&ctx->tarr, b));
+        HIPCHK(hipMalloc(&ctx->eini, b));
+    }
+    HIPCHK(hipMalloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
+    HIPCHK(hipMalloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
+    HIPCHK(hipMalloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
+    HIPCHK(hipMalloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
+    HIPCHK(hipMalloc((void **)&ctx->mass_partial, kMassBlocks * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&ctx->area_d, ctx->ny * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->area_d, g->area, ctx->ny * sizeof(double), hipMemcpyHostToDevice));
+    if (ctx->prec) {
+        std::vector<RowGeo<double>> rows;
+        std::vector<FaceGeoY<double>> fys;
+        fill_geometry(ctx, rows, fys);
+        HIPCHK(hipMalloc(&ctx->row, rows.size() * sizeof(rows[0])));
+        HIPCHK(hipMalloc(&ctx->fy, fys.size() * sizeof(fys[0])));
+        HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
+    } else {
+        std::vector<RowGeo<float>> rows;
+        std::vector<FaceGeoY<float>> fys;
+        fill_geometry(ctx, rows, fys);
+        HIPCHK(hipMalloc(&ctx->row, rows.size() * sizeof(rows[0])));
+        HIPCHK(hipMalloc(&ctx->fy, fys.size() * sizeof(fys[0])));
+        HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
+    }
+    // zero state so that padding columns hold finite values
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(hipMemset(ctx->e[k], 0, b));
+        HIPCHK(hipMemset(ctx->qx[k], 0, b));
+        HIPCHK(hipMemset(ctx->qy[k], 0, b));
+    }
+    HIPCHK(hipMemset(ctx->D, 0, b));
+    HIPCHK(hipMemset(ctx->emax, 0, b));
+    Ctrl c = idle_ctrl(ctx);
+    for (int s = 0; s < 3; s++) HIPCHK(hipMemcpy(&ctx->ctrl[s], &c, sizeof(c), hipMemcpyHostToDevice));
+    if ((r = reset_dt_slots(ctx)) != THY_OK) return r;
+    return THY_OK;
+}
+
+void thy_destroy(thy_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
+                    ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
+                    ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *thy_last_error(const thy_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int thy_upload_state(thy_ctx *ctx, const double *s1, const double *s2) {
+    if (!ctx || !s1 || !s2) return THY_ERR_INVALID;
+    return ctx->prec ? upload_t<double>(ctx, s1, s2) : upload_t<float>(ctx, s1, s2);
+}
+
+int thy_download_state(thy_ctx *ctx, double *s1, double *s2) {
+    if (!ctx) return THY_ERR_INVALID;
+    return ctx->prec ? download_t<double>(ctx, s1, s2) : download_t<float>(ctx, s1, s2);
+}
+
+int thy_init_metrics(thy_ctx *ctx) {
+    if (!ctx) return THY_ERR_INVALID;
+    return ctx->prec ? init_metrics_t<double>(ctx) : init_metrics_t<float>(ctx);
+}
+
+int thy_update_metrics(thy_ctx *ctx, double t) {
+    if (!ctx) return THY_ERR_INVALID;
+    return ctx->prec ? update_metrics_t<double>(ctx, t) : update_metrics_t<float>(ctx, t);
+}
+
+int thy_apply_okada(thy_ctx *ctx, const thy_fault *f, int crop_flag, double crop_value, double *def_out) {
+    if (!ctx || !f || crop_flag < 0 || crop_flag > 2) return THY_ERR_INVALID;
+    return ctx->prec ? apply_okada_t<double>(ctx, f, crop_flag, crop_value, def_out)
+                     : apply_okada_t<float>(ctx, f, crop_flag, crop_value, def_out);
+}
+
+int thy_initial_dt(thy_ctx *ctx, double *dt) {
+    if (!ctx || !dt) return THY_ERR_INVALID;
+    return ctx->prec ? initial_dt_t<double>(ctx, dt) : initial_dt_t<float>(ctx, dt);
+}
+
+int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
+    if (!ctx) return THY_ERR_INVALID;
+    int r;
+    Ctrl c = idle_ctrl(ctx);
+    c.dt = dt;
+    c.dt_valid = 1;
+    c.cur = ctx->cur;
+    if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
+    unsigned int big = 0x7f7fffffu;
+    const int s = ctx->slot;
+    HIPCHK(hipMemcpy(&ctx->dtmin[s], &big, sizeof(big), hipMemcpyHostToDevice));
+    hipError_t e = launch_step(ctx);
+    if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    Ctrl o;
+    HIPCHK(hipMemcpy(&o, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+    ctx->cur = o.cur;
+    if (dt_next) {
+        unsigned int v;
+        HIPCHK(hipMemcpy(&v, &ctx->dtmin[s], sizeof(v), hipMemcpyDeviceToHost));
+        float f;
+        memcpy(&f, &v, 4);
+        *dt_next = (double)f;
+    }
+    return THY_OK;
+}
+
+int thy_get_field(thy_ctx *ctx, int field, float *out) {
+    if (!ctx || !out || field < 0 || field > 5) return THY_ERR_INVALID;
+    return ctx->prec ? get_field_t<double>(ctx, field, out) : get_field_t<float>(ctx, field, out);
+}
+
+int thy_mass(thy_ctx *ctx, double *mass) {
+    if (!ctx || !mass) return THY_ERR_INVALID;
+    return ctx->prec ? mass_t<double>(ctx, mass) : mass_t<float>(ctx, mass);
+}
+
+int thy_last_timing(thy_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return THY_ERR_INVALID;
+    *ms = ctx->last_ms;
+    return THY_OK;
+}
+
+int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v, float *eta_min, float *eta_max,
+                   float *bathy) {
+    if (!ctx) return THY_ERR_INVALID;
+    const int ns = ctx->nsamples, ng = ctx->ng;
+    if (time) std::copy(ctx->g_time.begin(), ctx->g_time.begin() + ns, time);
+    if (eta) std::copy(ctx->g_eta.begin(), ctx->g_eta.begin() + (size_t)ns * ng, eta);
+    if (u) std::copy(ctx->g_u.begin(), ctx->g_u.begin() + (size_t)ns * ng, u);
+    if (v) std::copy(ctx->g_v.begin(), ctx->g_v.begin() + (size_t)ns * ng, v);
+    std::vector<int> pos(ng);
+    if (ng) HIPCHK(hipMemcpy(pos.data(), ctx->gpos, ng * sizeof(int), hipMemcpyDeviceToHost));
+    if (eta_min || eta_max) {
+        for (int g = 0; g < ng; g++) {
+            float mn = 1e30f, mx = -1e30f;
+            if (pos[g] < 0) mn = mx = -9999.0f;
+            else
+                for (int s = 0; s < ns; s++) {
+                    float x = ctx->g_eta[(size_t)s * ng + g];
+                    mn = std::min(x, mn);
+                    mx = std::max(x, mx);
+                }
+            if (eta_min) eta_min[g] = mn;
+            if (eta_max) eta_max[g] = mx;
+        }
+    }
+    if (bathy && ng) {
+        std::vector<float> field((size_t)ctx->nx * ctx->ny);
+        int r = thy_get_field(ctx, THY_DEFORMED_BATHY, field.data());
+        if (r != THY_OK) return r;
+        for (int g = 0; g < ng; g++) bathy[g] = pos[g] < 0 ? -9999.0f : field[pos[g]];
+    }
+    return THY_OK;
+}
+
+int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
+    if (!ctx || !cfg || !res) return THY_ERR_INVALID;
+    if (cfg->n_faults < 0 || (cfg->n_faults > 0 && !cfg->faults) || (cfg->n_gauges > 0 && !cfg->gauge_pos))
+        return fail(ctx, THY_ERR_INVALID, "bad run configuration");
+    int r;
+    memset(res, 0, sizeof(*res));
+    const int K = cfg->steps_per_sync > 0 ? ((cfg->steps_per_sync + 2) / 3) * 3 : 24;
+    // fault times
+    if (ctx->ftime) { (void)hipFree(ctx->ftime); ctx->ftime = nullptr; }
+    std::vector<double> ft(std::max(1, cfg->n_faults), 0.0);
+    for (int k = 0; k < cfg->n_faults; k++) ft[k] = cfg->faults[k].time;
+    HIPCHK(hipMalloc((void **)&ctx->ftime, ft.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->ftime, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
+    // gauges
+    if (ctx->gpos) { (void)hipFree(ctx->gpos); ctx->gpos = nullptr; }
+    if (ctx->gbuf) { (void)hipFree(ctx->gbuf); ctx->gbuf = nullptr; }
+    if (ctx->gtime) { (void)hipFree(ctx->gtime); ctx->gtime = nullptr; }
+    ctx->ng = cfg->n_gauges;
+    ctx->gcap = 0;
+    if (cfg->n_gauges > 0 && cfg->series_interval >= 0.0) {
+        double nsd = cfg->series_interval > 0.0 ? cfg->t_end / cfg->series_interval + 4.0 : 1e6;
+        if (cfg->max_steps > 0) nsd = std::min(nsd, (double)cfg->max_steps + 2.0);
+        ctx->gcap = (int)std::min(nsd, 1e6);
+    }
+    HIPCHK(hipMalloc((void **)&ctx->gpos, std::max(1, ctx->ng) * sizeof(int)));
+    if (ctx->ng) HIPCHK(hipMemcpy(ctx->gpos, cfg->gauge_pos, ctx->ng * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&ctx->gbuf, std::max<size_t>(1, (size_t)ctx->gcap * ctx->ng * 3) * sizeof(float)));
+    HIPCHK(hipMalloc((void **)&ctx->gtime, std::max(1, ctx->gcap) * sizeof(float)));
+    if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; ctx->gk = 0; }
+
+    // src/GPU/ShallowWater.cu:305-309: metrics from the pre-deformation state
+    if ((r = thy_init_metrics(ctx)) != THY_OK) return r;
+    // src/GPU/ShallowWater.cu:378-391: initial dt (pre-deformation) and truncation
+    double dt0;
+    if ((r = thy_initial_dt(ctx, &dt0)) != THY_OK) return r;
+    res->dt_initial = dt0;
+    Ctrl c = idle_ctrl(ctx);
+    c.t = 0.0;
+    c.t_end = cfg->t_end;
+    c.max_steps = cfg->max_steps;
+    c.nf = cfg->n_faults;
+    c.k = 0;
+    c.T = ctx->prm.T;
+    c.next_save = cfg->save_interval >= 0.0 ? 0.0 : INFINITY;
+    c.next_series = (cfg->series_interval >= 0.0 && ctx->ng > 0) ? 0.0 : INFINITY;
+    c.series_dt = cfg->series_interval;
+    c.gcap = ctx->gcap;
+    double dt = dt0;
+    int skip = 0;
+    if (cfg->n_faults > 0) {
+        auto T = ctx->prm.T;
+        if (ft[0] < 2.220446049250313e-16) {
+            // skipping variant (src/GPU/Timestep.cu:46-68)
+            int i = 1;
+            while (i < cfg->n_faults && !(std::fabs(ft[i] - ft[0]) > 2.220446049250313e-16)) i++;
+            if (i < cfg->n_faults && (0.0 + dt) * T > ft[i]) { dt = ft[i] / T - 0.0; skip = 1; }
+        } else if ((0.0 + dt) * T > ft[0]) {
+            dt = ft[0] / T - 0.0;
+            skip = 1;
+        }
+    }
+    c.dt = dt;
+    c.dt_valid = 1;
+    c.skip = skip;
+    c.cur = ctx->cur;
+    if ((r = reset_dt_slots(ctx)) != THY_OK) return r;
+    if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
+    double next_save = c.next_save;
+    int nsnap = 0;
+    double total_ms = 0.0;
+    for (;;) {
+        Ctrl s;
+        double ms = 0.0;
+        if ((r = advance(ctx, K, &s, &ms)) != THY_OK) return r;
+        total_ms += ms;
+        if (!(s.t < s.t_end) || (s.max_steps > 0 && s.steps >= s.max_steps)) {
+            res->steps = s.steps;
+            res->t_final = s.t;
+            ctx->nsamples = std::min(s.nsamples, ctx->gcap);
+            res->n_samples = s.nsamples;
+            break;
+        }
+        bool changed = false;
+        // src/GPU/ShallowWater.cu:125-147: apply every fault whose time has come
+        while (s.k < s.nf && s.t * s.T >= ft[s.k]) {
+            if (cfg->on_okada) cfg->on_okada(cfg->user, s.k);
+            if ((r = thy_apply_okada(ctx, &cfg->faults[s.k], cfg->crop_flag, cfg->crop_value, nullptr)) != THY_OK)
+                return r;
+            s.k++;
+            changed = true;
+        }
+        // src/GPU/ShallowWater.cu:404-409: snapshot
+        if (s.t >= next_save) {
+            next_save += cfg->save_interval;
+            s.next_save = next_save;
+            if (cfg->on_snapshot) cfg->on_snapshot(cfg->user, nsnap, s.t, ctx);
+            nsnap++;
+            changed = true;
+        }
+        if (!changed) return fail(ctx, THY_ERR_INVALID, "driver stalled");
+        if ((r = write_ctrl(ctx, s)) != THY_OK) return r;
+    }
+    res->n_snapshots = nsnap;
+    res->seconds = total_ms * 1e-3;
+    ctx->last_ms = total_ms;
+    // collect gauge samples
+    const int ns = ctx->nsamples, ng = ctx->ng;
+    ctx->g_time.assign(ns, 0.f);
+    ctx->g_eta.assign((size_t)ns * ng, 0.f);
+    ctx->g_u.assign((size_t)ns * ng, 0.f);
+    ctx->g_v.assign((size_t)ns * ng, 0.f);
+    if (ns > 0 && ng > 0) {
+        std::vector<float> buf((size_t)ns * ng * 3);
+        HIPCHK(hipMemcpy(buf.data(), ctx->gbuf, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ctx->g_time.data(), ctx->gtime, ns * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)ns * ng; i++) {
+            ctx->g_eta[i] = buf[3 * i];
+            ctx->g_u[i] = buf[3 * i + 1];
+            ctx->g_v[i] = buf[3 * i + 2];
+        }
+    }
+    return THY_OK;
+}
+
+}  // extern "C"

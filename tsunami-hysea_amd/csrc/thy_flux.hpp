@@ -1,0 +1,372 @@
+// thy_flux.hpp -- per-face 2s+WAF flux math of the Tsunami-HySEA scheme for gfx950.
+//
+// Restates the reference's edge functions (src/GPU/Arista_kernel.cu) for one face,
+// templated on the arithmetic type (float for the production path, double for the
+// FP64-everywhere precision study).  State is carried as
+//     e = eta relative to sea level (h - D),  D = still-water depth (unshifted),
+// instead of the reference's (h, H = D - Hmin).  Every place where the reference
+// forms eta_k + min(H0,H1) becomes e_k + min(D0,D1) (the Hmin shift cancels), and the
+// hydrostatic difference max(a,0) - max(b,0) with both sides wet is evaluated as
+// e1 - e0, which is the same number without the cancellation of two O(1) depths.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace thy {
+
+// Guards keep the reference's DBL_EPSILON (src/Constantes.hxx:11) in both precisions.
+template <typename T> struct Num;
+
+template <> struct Num<float> {
+    static constexpr float eps = 2.220446049250313e-16f;
+    static constexpr float big = 1e30f;
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float exp(float x) {
+        return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+    }
+    // h^(4/3) for h > 0
+    static __device__ __forceinline__ float pow43(float x) {
+        return __builtin_amdgcn_exp2f(1.3333333333333333f * __builtin_amdgcn_logf(x));
+    }
+};
+
+template <> struct Num<double> {
+    static constexpr double eps = 2.220446049250313e-16;
+    static constexpr double big = 1e30;
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double pow43(double x) { return ::pow(x, 4.0 / 3.0); }
+};
+
+template <typename T> struct Prm {
+    T eps_h, hpos, cvis, cd2s;   // cd2s = 0.5*(1-cvis)
+    T bl, bh;                    // border factors of the low / high side of this face direction
+};
+
+// One stencil cell: total depth h, eta e, depth D, normal and tangential discharge.
+template <typename T> struct SC {
+    T h, e, D, qn, qt;
+};
+
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+template <typename T> __device__ __forceinline__ T sgn(T x) {
+    return (x > T(0) ? T(1) : T(0)) - (x < T(0) ? T(1) : T(0));
+}
+
+// sqrt(2) h / sqrt(h^4 + max(h,eps)^4): src/GPU/Arista_kernel.cu:163-168
+template <typename T> __device__ __forceinline__ T desing(T h, T eps_h) {
+    T m = tmax(h, eps_h);
+    T h2 = h * h, m2 = m * m;
+    return Num<T>::div(T(1.4142135623730951) * h, Num<T>::sqrt(h2 * h2 + m2 * m2));
+}
+
+// smooth monotone limiter: src/GPU/Arista_kernel.cu:45-68
+template <typename T> __device__ __forceinline__ T limiter(T p0m, T p0, T p1, T p1p, T S) {
+    T c0 = p1 - p0;
+    T c1 = (S < T(0)) ? (p1p - p1) : ((S > T(0)) ? (p0 - p0m) : T(0.5) * (p1p - p0m));
+    T sg = (c0 * c1 >= T(0)) ? T(1) : T(0);
+    c0 = tabs(c0);
+    c1 = tabs(c1);
+    if (c0 < Num<T>::eps) return T(1);
+    T aux = tabs(Num<T>::div(c1 - c0, c1 + c0));
+    c1 = T(0.5) * (c1 + c0) * (T(1) - aux * aux * aux);
+    T vs = (c0 > c1) ? Num<T>::div(c1, c0 + Num<T>::eps) : Num<T>::div(c0, c1 + Num<T>::eps);
+    return Num<T>::div(sg * vs * (T(1) + vs), T(1) + vs * vs);
+}
+
+// max(e1 + Dm, 0) - max(e0 + Dm, 0) with Dm = min(D0, D1)  (src/GPU/Arista_kernel.cu:201-202)
+template <typename T> __device__ __forceinline__ T deta_of(T e0, T e1, T D0, T D1) {
+    T Dm = tmin(D0, D1);
+    T a = e1 + Dm, b = e0 + Dm;
+    return (a > T(0) && b > T(0)) ? (e1 - e0) : (tmax(a, T(0)) - tmax(b, T(0)));
+}
+
+// 2s/WAF blend: src/GPU/Arista_kernel.cu:205-208
+template <typename T> __device__ __forceinline__ T blend(T hij, T hpos, bool alfa_cero) {
+    if (alfa_cero) return T(0);
+    if (hij < hpos)
+        return Num<T>::rcp(T(1) + Num<T>::exp(T(-15) * (T(1) - Num<T>::div(T(0.5) * hpos, hij + Num<T>::eps))));
+    return T(1);
+}
+
+// positivity limiter: src/GPU/Arista_kernel.cu:79-114 (g0/g1 = 1 for x faces, cos for y)
+template <typename T> __device__ __forceinline__ T positivity(T F, T h0, T h1, T g0, T g1, T dt, T A0, T A1) {
+    T dt0 = (F > T(0)) ? Num<T>::div(T(0.25) * h0 * g0 * A0, F + Num<T>::eps)
+                       : Num<T>::div(T(0.25) * h1 * g1 * A1, -F + Num<T>::eps);
+    if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
+    return F;
+}
+
+// wet/dry treatment of the normal discharges: src/GPU/Arista_kernel.cu:70-77
+template <typename T> __device__ __forceinline__ void wet_dry(const SC<T> &c0, const SC<T> &c1, T eps_h, T &q0, T &q1) {
+    if (c0.h < eps_h && -c1.e > c0.D) q1 = T(0);
+    if (c1.h < eps_h && -c0.e > c1.D) q0 = T(0);
+}
+
+// Apply the ghost-cell border of a face and orient it: border 1 = low-side border
+// (ghost on the low side, swapped so the interior cell is "0", normal -1),
+// border 2 = high-side border.  src/GPU/Arista_kernel.cu:506-524,550-556
+template <typename T> __device__ __forceinline__ T orient(int border, const Prm<T> &p, SC<T> &m, SC<T> &a, SC<T> &b, SC<T> &q) {
+    if (border == 1) {
+        a.qn *= p.bl;
+        SC<T> t = a; a = b; b = t;
+        t = m; m = q; q = t;
+        return T(-1);
+    }
+    if (border == 2) b.qn *= p.bh;
+    return T(1);
+}
+
+// ---------------------------------------------------------------------------
+// Mass (step 1) face flux.  Returns F: low cell -= F, high cell += F (x); the y
+// caller divides by the cell's cos.  Border faces return the value that the
+// single adjacent interior cell receives with the sign convention documented in
+// the step kernel.
+//   x: src/GPU/Arista_kernel.cu:118-219,466-563   y: :674-778,1028-1125
+// g0/g1: 1/cos(phi_row) for x faces (both sides), cos0/cos1 for y faces.
+// ---------------------------------------------------------------------------
+template <typename T, bool YFACE>
+__device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int border, bool alfa_cero,
+                                       T len, T g0, T g1, T A0, T A1, T valf, T dt, const Prm<T> &p) {
+    // deep branch (2s): all four depths above hpos and outside the first-order band
+    if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
+        T F = YFACE ? len * a.qn * g0 : a.qn * len * g0;
+        return YFACE ? positivity(F, a.h, b.h, g0, g1, dt, A0, A1) : positivity(F, a.h, b.h, T(1), T(1), dt, A0, A1);
+    }
+    T normal = orient(border, p, m, a, b, q);
+    if (border == 1) { T t = g0; g0 = g1; g1 = t; t = A0; A0 = A1; A1 = t; }
+    if (!(a.h > Num<T>::eps || b.h > Num<T>::eps)) return T(0);
+    T q0 = a.qn * normal, q1 = b.qn * normal;
+    T hij = T(0.5) * (a.h + b.h);
+    wet_dry(a, b, p.eps_h, q0, q1);
+    T u0 = desing(a.h, p.eps_h) * q0;
+    T u1 = desing(b.h, p.eps_h) * q1;
+    q0 = u0 * a.h;
+    q1 = u1 * b.h;
+    T s0 = Num<T>::sqrt(a.h), s1 = Num<T>::sqrt(b.h);
+    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T val = valf * dt;   // 2*len*dt/(A0+A1) with len/cos for x
+    T c0 = Num<T>::sqrt(hij);
+    T l1 = tmin(uij - c0, u0 - s0);
+    T l2 = tmax(uij + c0, u1 + s1);
+    T lmax = tmax(tabs(l2), tabs(l1));
+    T xiL = T(0), xiR = T(0);
+    if (!alfa_cero) {
+        xiL = limiter(m.e, a.e, b.e, q.e, l1);
+        xiR = limiter(m.e, a.e, b.e, q.e, l2);
+    }
+    T a0;
+    if (tabs(l1 - l2) >= Num<T>::eps) {
+        T alr = sgn(l1) * (T(1) - xiL) + val * l1 * xiL;
+        T blr = sgn(l2) * (T(1) - xiR) + val * l2 * xiR;
+        a0 = Num<T>::div((alr - blr) * (l1 * l2), l2 - l1);
+    } else {
+        a0 = lmax;
+    }
+    T F2s, Fw;
+    if (YFACE) {
+        F2s = len * q0 * g0 * normal;
+        T deta = T(0.5) * (g0 + g1) * deta_of(a.e, b.e, a.D, b.D) - uij * uij * (g1 - g0);
+        Fw = T(0.5) * (q0 * g0 + q1 * g1 - a0 * deta) * len;
+    } else {
+        T il = len * g0;
+        F2s = il * q0 * normal;
+        T deta = deta_of(a.e, b.e, a.D, b.D);
+        Fw = T(0.5) * (q1 + q0 - a0 * deta) * il;
+    }
+    T alfa = blend(hij, p.hpos, alfa_cero);
+    T F = alfa * F2s + (T(1) - alfa) * Fw;
+    return YFACE ? positivity(F, a.h, b.h, g0, g1, dt, A0, A1) : positivity(F, a.h, b.h, T(1), T(1), dt, A0, A1);
+}
+
+// Momentum face result: low cell acc2 -= (mn, mt), high cell acc2 -= (pn, pt),
+// both cells acc1.y += w.  (n = normal component, t = tangential.)
+template <typename T> struct MF {
+    T mn, mt, pn, pt, w;
+};
+
+// ---------------------------------------------------------------------------
+// Momentum (step 2) face flux.
+//   x: src/GPU/Arista_kernel.cu:221-464,565-670   y: :780-1026,1127-1233
+// For x faces: g0 = g1 = 1/cos(phi_row) (len*g = len/cos), qn = qx, qt = qy.
+// For y faces: g0 = cos0, g1 = cos1, qn = qy, qt = qx.
+// ---------------------------------------------------------------------------
+template <typename T, bool YFACE>
+__device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int border, bool alfa_cero,
+                                          T len, T g0, T g1, T valf, T dt, const Prm<T> &p) {
+    MF<T> o;
+    if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
+        // deep-water 2s branch
+        T hij = T(0.5) * (a.h + b.h);
+        T ia = Num<T>::rcp(a.h), ib = Num<T>::rcp(b.h);
+        T u0 = a.qn * ia, u1 = b.qn * ib;
+        T s0 = Num<T>::sqrt(a.h), s1 = Num<T>::sqrt(b.h);
+        T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+        T lmax = tabs(uij) + Num<T>::sqrt(hij);
+        T deta = deta_of(a.e, b.e, a.D, b.D);
+        T val = valf * dt;
+        bool up = uij < T(0);
+        T uu = up ? u1 : u0;
+        T qnn = up ? b.qn : a.qn;
+        T qtt = up ? b.qt : a.qt;
+        if (YFACE) {
+            T ic0 = Num<T>::rcp(g0), ic1 = Num<T>::rcp(g1);
+            deta = T(0.5) * (g0 + g1) * deta - uij * uij * (g1 - g0);
+            T fn = uu * qnn * len;
+            T mn, pn;
+            if (up) { pn = -fn; mn = fn * (g1 * ic0); }
+            else { mn = fn; pn = -fn * g0 * ic1; }
+            mn += hij * deta * len * ic0;
+            T DES = p.cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
+            o.mn = mn - DES * ic0;
+            o.pn = pn + DES * ic1;
+            T ft = uu * qtt * len;
+            o.mt = ft;
+            o.pt = -ft;
+            o.w = len * lmax;
+        } else {
+            T il = len * g0;
+            T fn = uu * qnn * il;
+            T DES = p.cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
+            o.mn = fn + hij * deta * il - DES;
+            o.pn = -fn + DES;
+            T ft = uu * qtt * il;
+            o.mt = ft;
+            o.pt = -ft;
+            o.w = il * lmax;
+        }
+        return o;
+    }
+    T normal = orient(border, p, m, a, b, q);
+    if (border == 1) { T t = g0; g0 = g1; g1 = t; }
+    if (!(a.h > Num<T>::eps || b.h > Num<T>::eps)) {
+        o.mn = o.mt = o.pn = o.pt = o.w = T(0);
+        return o;
+    }
+    T ic0 = YFACE ? Num<T>::rcp(g0) : T(1), ic1 = YFACE ? Num<T>::rcp(g1) : T(1);
+    T L = YFACE ? len : len * g0;   // face length factor (len/cos for x)
+    T h0 = a.h, h1 = b.h;
+    T q0 = a.qn * normal, q1 = b.qn * normal;
+    T hij = T(0.5) * (h0 + h1);
+    wet_dry(a, b, p.eps_h, q0, q1);
+    T f0 = desing(h0, p.eps_h), f1 = desing(h1, p.eps_h);
+    T u0 = f0 * q0, u1 = f1 * q1;
+    if (!YFACE) { q0 = u0 * h0; q1 = u1 * h1; }
+    T s0 = Num<T>::sqrt(h0), s1 = Num<T>::sqrt(h1);
+    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T val = valf * dt;
+    T c0 = Num<T>::sqrt(hij);
+    T l1 = tmin(uij - c0, u0 - s0);
+    T l2 = tmax(uij + c0, u1 + s1);
+    T lmax = tmax(tabs(l2), tabs(l1));
+    T deta = deta_of(a.e, b.e, a.D, b.D);
+    T fl0x, fl0y, fl1x, fl1y;
+    if (YFACE) {
+        deta = T(0.5) * (g0 + g1) * deta - uij * uij * (g1 - g0);
+        fl1x = q1 * g1; fl1y = u1 * fl1x;
+        fl0x = q0 * g0; fl0y = u0 * fl0x;
+    } else {
+        fl1x = q1; fl1y = u1 * q1;
+        fl0x = q0; fl0y = u0 * q0;
+    }
+    T dq = fl1x - fl0x;
+    // 2s part (s: normal momentum, z: tangential)
+    bool up = uij < T(0);
+    T sm_n, sp_n, sm_t, sp_t;
+    {
+        T uu = up ? u1 : u0;
+        T qq = up ? q1 : q0;
+        T qt = up ? b.qt : a.qt;
+        T fn = uu * qq * L * normal;
+        if (YFACE) {
+            if (up) { sp_n = -fn; sm_n = fn * (g1 * ic0); }
+            else { sm_n = fn; sp_n = -fn * g0 * ic1; }
+            sm_n += hij * deta * L * ic0;
+        } else {
+            sp_n = -fn;
+            sm_n = fn + hij * deta * L;
+        }
+        T ft = uu * qt * L;
+        sm_t = ft;
+        sp_t = -ft;
+        T cd = T(0.5) * (T(1) - Num<T>::div(p.cvis, T(1) + Num<T>::exp(T(-20) * (T(1) - Num<T>::div(p.hpos, hij)))));
+        T des = cd * val * lmax * lmax * dq * L;
+        if (YFACE) { sm_n -= des * ic0; sp_n += des * ic1; }
+        else { des *= normal; sm_n -= des; sp_n += des; }
+    }
+    // WAF part
+    T xiL = T(0), xiR = T(0);
+    if (!alfa_cero) {
+        xiL = limiter(m.e, a.e, b.e, q.e, l1);
+        xiR = limiter(m.e, a.e, b.e, q.e, l2);
+    }
+    T a0, a1;
+    if (tabs(l1 - l2) >= Num<T>::eps) {
+        T alr = sgn(l1) * (T(1) - xiL) + val * l1 * xiL;
+        T blr = sgn(l2) * (T(1) - xiR) + val * l2 * xiR;
+        T aux = Num<T>::rcp(l2 - l1);
+        a0 = (alr - blr) * (l1 * l2) * aux;
+        a1 = (blr * l2 - alr * l1) * aux;
+    } else {
+        a0 = lmax;
+        a1 = T(0);
+    }
+    T wy = T(0.5) * (fl1y - fl0y + hij * deta);
+    T DESx = T(0.5) * a0 * deta;
+    T DESy = T(0.5) * a0 * dq + a1 * wy;
+    T mxw = (T(0.5) * dq + (fl0x - DESx)) * L;
+    T myw = (wy + (fl0y - DESy)) * L;
+    T pyw = (wy + (DESy - fl1y)) * L;
+    // limited upwinding of the tangential velocity
+    T tsign = YFACE ? T(-1) : T(1);
+    T q0mt = tsign * m.qt * normal, q0t = tsign * a.qt * normal;
+    T q1t = tsign * b.qt * normal, q1pt = tsign * q.qt * normal;
+    T p0m = desing(m.h, p.eps_h) * q0mt;
+    T p0 = f0 * q0t;
+    T p1 = f1 * q1t;
+    T p1p = desing(q.h, p.eps_h) * q1pt;
+    T sgm = sgn(mxw);
+    T us = tabs(uij) * sgm;
+    T xi = alfa_cero ? T(0) : limiter(p0m, p0, p1, p1p, us);
+    T uijt;
+    if (YFACE) uijt = T(0.5) * (p0 * ic0 + p1 * ic1) - T(0.5) * ((T(1) - xi) * sgm + xi * us * val) * (p1 * ic1 - p0 * ic0);
+    else uijt = T(0.5) * (p0 + p1) - T(0.5) * ((T(1) - xi) * sgm + xi * us * val) * (p1 - p0);
+    T aux = mxw * uijt;
+    T f3m_n, f3m_t, f3p_n, f3p_t;
+    if (YFACE) {
+        f3p_t = aux * normal;  f3p_n = pyw * normal * ic1;
+        f3m_t = -aux * normal; f3m_n = myw * normal * ic0;
+    } else {
+        f3p_n = pyw * normal;  f3p_t = -aux * normal;
+        f3m_n = myw * normal;  f3m_t = aux * normal;
+    }
+    T alfa = blend(hij, p.hpos, alfa_cero);
+    T oma = T(1) - alfa;
+    o.mn = alfa * sm_n + oma * f3m_n;
+    o.mt = alfa * sm_t + oma * f3m_t;
+    o.pn = alfa * sp_n + oma * f3p_n;
+    o.pt = alfa * sp_t + oma * f3p_t;
+    if (lmax < p.eps_h) lmax += p.eps_h;
+    o.w = L * lmax;
+    return o;
+}
+
+// Initial-dt wave speed across a face: src/GPU/Arista_kernel.cu:1237-1280
+template <typename T>
+__device__ __forceinline__ T init_wave(T h0, T q0, T h1, T q1, T eps_h) {
+    T hij = T(0.5) * (h0 + h1);
+    T u0 = desing(h0, eps_h) * q0;
+    T u1 = desing(h1, eps_h) * q1;
+    T s0 = Num<T>::sqrt(h0), s1 = Num<T>::sqrt(h1);
+    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T lmax = tabs(uij) + Num<T>::sqrt(hij);
+    if (lmax < eps_h) lmax += eps_h;
+    return lmax;
+}
+
+}  // namespace thy

@@ -1,0 +1,237 @@
+"""ctypes binding of libthysea.so (C-ABI in include/thysea.h).
+
+The HIP library is the compute path.  There is no fallback: if the library is not
+built, or no gfx950 device is present, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libthysea.so")
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+FIELDS = {"eta": 0, "ux": 1, "uy": 2, "deformed_bathy": 3, "max_height": 4, "arrival_times": 5}
+STATUS = {0: "ok", 1: "device memory", 2: "host memory", 3: "invalid argument", 4: "no device",
+          5: "HIP error", 6: "memory budget", 7: "non-finite state"}
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("lon", _dp), ("lat", _dp), ("area", _dp),
+                ("cosphi", _dp), ("ancho", _dp), ("alto", _dp)]
+
+
+class Params(C.Structure):
+    _fields_ = [("borde_sup", C.c_double), ("borde_inf", C.c_double), ("borde_izq", C.c_double),
+                ("borde_der", C.c_double), ("sponge_sup", C.c_int), ("sponge_inf", C.c_int),
+                ("sponge_izq", C.c_int), ("sponge_der", C.c_int), ("cfl", C.c_double), ("mf0", C.c_double),
+                ("vmax", C.c_double), ("eps_h", C.c_double), ("hpos", C.c_double), ("cvis", C.c_double),
+                ("dif_at", C.c_double), ("hmin", C.c_double), ("sea_level", C.c_double), ("L", C.c_double),
+                ("H", C.c_double), ("Q", C.c_double), ("T", C.c_double), ("precision", C.c_int),
+                ("track_arrival", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int)]
+
+
+class Fault(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("time", "lon_c", "lat_c", "depth_c", "length", "width", "strike", "dip", "rake", "slip")]
+
+
+SNAPFN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_void_p)
+OKADAFN = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+
+
+class RunConfig(C.Structure):
+    _fields_ = [("t_end", C.c_double), ("save_interval", C.c_double), ("series_interval", C.c_double),
+                ("n_faults", C.c_int), ("faults", C.POINTER(Fault)), ("crop_flag", C.c_int),
+                ("crop_value", C.c_double), ("n_gauges", C.c_int), ("gauge_pos", _ip),
+                ("max_steps", C.c_int64), ("steps_per_sync", C.c_int), ("on_snapshot", SNAPFN),
+                ("on_okada", OKADAFN), ("user", C.c_void_p)]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("n_samples", C.c_int), ("n_snapshots", C.c_int),
+                ("t_final", C.c_double), ("dt_initial", C.c_double), ("seconds", C.c_double)]
+
+
+class ThyError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ThyError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.thy_probe.restype = C.c_int
+        L.thy_create.argtypes = [C.POINTER(Grid), C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.thy_destroy.argtypes = [C.c_void_p]
+        L.thy_last_error.argtypes = [C.c_void_p]
+        L.thy_last_error.restype = C.c_char_p
+        L.thy_upload_state.argtypes = [C.c_void_p, _dp, _dp]
+        L.thy_download_state.argtypes = [C.c_void_p, _dp, _dp]
+        L.thy_init_metrics.argtypes = [C.c_void_p]
+        L.thy_update_metrics.argtypes = [C.c_void_p, C.c_double]
+        L.thy_apply_okada.argtypes = [C.c_void_p, C.POINTER(Fault), C.c_int, C.c_double, _dp]
+        L.thy_initial_dt.argtypes = [C.c_void_p, _dp]
+        L.thy_step.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.thy_run.argtypes = [C.c_void_p, C.POINTER(RunConfig), C.POINTER(RunResult)]
+        L.thy_get_field.argtypes = [C.c_void_p, C.c_int, _fp]
+        L.thy_get_gauges.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.thy_mass.argtypes = [C.c_void_p, _dp]
+        L.thy_footprint.argtypes = [C.c_int, C.c_int, C.POINTER(Params)]
+        L.thy_footprint.restype = C.c_size_t
+        L.thy_last_timing.argtypes = [C.c_void_p, _dp]
+        _LIB = L
+    return _LIB
+
+
+def probe():
+    return lib().thy_probe()
+
+
+def params_of(prob, precision=0, track_arrival=True, tile=(0, 0)):
+    p = Params()
+    p.borde_sup, p.borde_inf, p.borde_izq, p.borde_der = prob.borde_sup, prob.borde_inf, prob.borde_izq, prob.borde_der
+    p.sponge_sup, p.sponge_inf = prob.sponge_sup, prob.sponge_inf
+    p.sponge_izq, p.sponge_der = prob.sponge_izq, prob.sponge_der
+    p.cfl, p.mf0, p.vmax = prob.cfl, prob.mf0, prob.vmax
+    p.eps_h, p.hpos, p.cvis, p.dif_at = prob.eps_h, prob.hpos, prob.cvis, prob.dif_at
+    p.hmin, p.sea_level = prob.hmin, 0.0
+    p.L, p.H, p.Q, p.T = prob.L, prob.H_ref, prob.Q, prob.T
+    p.precision = int(precision)
+    p.track_arrival = 1 if track_arrival else 0
+    p.tile_x, p.tile_y = int(tile[0]), int(tile[1])
+    return p
+
+
+def fault_of(f):
+    out = Fault()
+    for k, _ in Fault._fields_:
+        setattr(out, k, float(f[k]))
+    return out
+
+
+class Solver:
+    """One grid on the GPU, driven through the C-ABI."""
+
+    def __init__(self, prob, precision=0, track_arrival=True, tile=(0, 0), upload=True):
+        self.prob = prob
+        self._keep = [np.ascontiguousarray(a, np.float64) for a in
+                      (prob.lon, prob.lat, prob.area, prob.cosphi, prob.ancho, prob.alto)]
+        g = Grid(prob.nx, prob.ny, *[a.ctypes.data_as(_dp) for a in self._keep])
+        self.params = params_of(prob, precision, track_arrival, tile)
+        h = C.c_void_p()
+        rc = lib().thy_create(C.byref(g), C.byref(self.params), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().thy_last_error(h).decode() if h.value else ""
+            self.close()
+            raise ThyError(f"thy_create: {STATUS.get(rc, rc)} {msg}")
+        self.n = prob.nx * prob.ny
+        if upload:
+            self.upload(prob.h, prob.H, prob.qx, prob.qy)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {lib().thy_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().thy_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, h, H, qx, qy):
+        s1 = np.empty((self.n, 2))
+        s2 = np.empty((self.n, 2))
+        s1[:, 0] = np.ravel(h)
+        s1[:, 1] = np.ravel(H)
+        s2[:, 0] = np.ravel(qx)
+        s2[:, 1] = np.ravel(qy)
+        self._chk(lib().thy_upload_state(self.h, s1.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)), "upload")
+
+    def state(self):
+        s1 = np.empty((self.n, 2))
+        s2 = np.empty((self.n, 2))
+        self._chk(lib().thy_download_state(self.h, s1.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)), "download")
+        ny, nx = self.prob.ny, self.prob.nx
+        return (s1[:, 0].reshape(ny, nx), s1[:, 1].reshape(ny, nx),
+                s2[:, 0].reshape(ny, nx), s2[:, 1].reshape(ny, nx))
+
+    def init_metrics(self):
+        self._chk(lib().thy_init_metrics(self.h), "init_metrics")
+
+    def update_metrics(self, t):
+        self._chk(lib().thy_update_metrics(self.h, float(t)), "update_metrics")
+
+    def okada(self, f, crop_flag=0, crop_value=0.0, want_def=False):
+        d = np.empty(self.n) if want_def else None
+        ff = fault_of(f)
+        self._chk(lib().thy_apply_okada(self.h, C.byref(ff), int(crop_flag), float(crop_value),
+                                        d.ctypes.data_as(_dp) if want_def else None), "okada")
+        return None if d is None else d.reshape(self.prob.ny, self.prob.nx)
+
+    def initial_dt(self):
+        v = C.c_double()
+        self._chk(lib().thy_initial_dt(self.h, C.byref(v)), "initial_dt")
+        return v.value
+
+    def step(self, dt):
+        v = C.c_double()
+        self._chk(lib().thy_step(self.h, float(dt), C.byref(v)), "step")
+        return v.value
+
+    def field(self, name):
+        out = np.empty(self.n, np.float32)
+        self._chk(lib().thy_get_field(self.h, FIELDS[name], out.ctypes.data_as(_fp)), "get_field")
+        return out.reshape(self.prob.ny, self.prob.nx)
+
+    def mass(self):
+        v = C.c_double()
+        self._chk(lib().thy_mass(self.h, C.byref(v)), "mass")
+        return v.value
+
+    def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None):
+        pr = self.prob
+        faults = sorted(pr.faults if faults is None else faults, key=lambda f: f["time"])
+        farr = (Fault * max(1, len(faults)))(*[fault_of(f) for f in faults])
+        gpos = np.ascontiguousarray(pr.gauge_pos if pr.gauge_pos is not None else np.zeros(0), np.int32)
+        c = RunConfig()
+        c.t_end, c.save_interval, c.series_interval = pr.t_end, pr.save_interval, pr.series_interval
+        c.n_faults, c.faults = len(faults), farr
+        c.crop_flag, c.crop_value = pr.crop_flag, pr.crop_value
+        c.n_gauges, c.gauge_pos = len(gpos), gpos.ctypes.data_as(_ip)
+        c.max_steps, c.steps_per_sync = int(max_steps), int(steps_per_sync)
+        snap = SNAPFN(lambda u, i, t, ctx: on_snapshot(i, t)) if on_snapshot else SNAPFN()
+        oka = OKADAFN(lambda u, k: on_okada(k)) if on_okada else OKADAFN()
+        c.on_snapshot, c.on_okada = snap, oka
+        r = RunResult()
+        self._chk(lib().thy_run(self.h, C.byref(c), C.byref(r)), "run")
+        ng = len(gpos)
+        ns = r.n_samples
+        bufs = [np.zeros(max(1, ns), np.float32)] + [np.zeros(max(1, ns * ng), np.float32) for _ in range(3)] + \
+               [np.zeros(max(1, ng), np.float32) for _ in range(3)]
+        self._chk(lib().thy_get_gauges(self.h, *[b.ctypes.data_as(_fp) for b in bufs]), "get_gauges")
+        return {"steps": r.steps, "t_final": r.t_final, "dt_initial": r.dt_initial, "seconds": r.seconds,
+                "n_samples": ns, "n_snapshots": r.n_snapshots, "g_time": bufs[0][:ns],
+                "g_eta": bufs[1][:ns * ng].reshape(ns, ng), "g_u": bufs[2][:ns * ng].reshape(ns, ng),
+                "g_v": bufs[3][:ns * ng].reshape(ns, ng), "g_eta_min": bufs[4][:ng], "g_eta_max": bufs[5][:ng],
+                "g_bathy": bufs[6][:ng]}
+
+
+def footprint(nx, ny, precision=0, track_arrival=True):
+    p = Params()
+    p.precision = precision
+    p.track_arrival = 1 if track_arrival else 0
+    return lib().thy_footprint(nx, ny, C.byref(p))
