@@ -71,6 +71,10 @@ typedef struct thy_fault {
 /* Per-run schedule for thy_run (mirrors the loop of src/GPU/ShallowWater.cu:374-453). */
 typedef void (*thy_snapshot_fn)(void *user, int index, double t_nondim, thy_ctx *ctx);
 typedef void (*thy_okada_fn)(void *user, int fault_index);
+/* Called once per finished iteration (1-based), in order, with the dt chosen for the
+   next iteration and the time reached (non-dimensional), as the reference prints them
+   (src/GPU/ShallowWater.cu:447-448).  Delivered after each graph replay. */
+typedef void (*thy_step_fn)(void *user, int64_t iteration, double dt_next, double t_nondim);
 
 typedef struct thy_run_config {
     double t_end;           /* tiempo_tot (non-dimensional) */
@@ -86,6 +90,7 @@ typedef struct thy_run_config {
     int steps_per_sync;     /* steps per captured graph replay (0: default) */
     thy_snapshot_fn on_snapshot; /* called at each NetCDF snapshot time */
     thy_okada_fn on_okada;       /* called before each Okada application */
+    thy_step_fn on_step;         /* per-iteration log (may be NULL) */
     void *user;
 } thy_run_config;
 
@@ -145,6 +150,11 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next);
    device-resident t and dt; the host synchronises once per replay. */
 int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res);
 
+/* Continue the schedule of the last thy_run for up to n_steps more steps (or until
+   its end time).  No reference counterpart: the reference loop cannot be resumed;
+   this lets a caller time an exact number of steps after a warm-up. */
+int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res);
+
 /* Output field conversion to float (src/GPU/netcdf_kernel.cu), [ny*nx] row-major. */
 int thy_get_field(thy_ctx *ctx, int field, float *out);
 
@@ -153,11 +163,16 @@ int thy_get_field(thy_ctx *ctx, int field, float *out);
 int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v,
                    float *eta_min, float *eta_max, float *bathy);
 
-/* Compensated FP64 sum of h*area (mass), for drift checks. */
+/* Compensated FP64 sum of h*area*cos(phi) (the scheme's mass invariant), for drift checks. */
 int thy_mass(thy_ctx *ctx, double *mass);
 
 /* Device bytes held by one context of this grid and parameters (admission). */
 size_t thy_footprint(int nx, int ny, const thy_params *params);
+
+/* Plain device-to-device copy kernel of `bytes` bytes run `reps` times; returns the
+   time per copy.  No reference counterpart: it calibrates HBM counters and the
+   achievable copy bandwidth beside the step kernel. */
+int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 /* Wall time (ms) of the last kernel launch group, for benchmarking. */
 int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);

@@ -1476,11 +1476,14 @@ void orc_gauges(const orc_sim *s, const int *pos, int n, double H, double Q,
 
 double orc_mass(const orc_sim *s)
 {
-    /* Neumaier-compensated sum of h*area */
+    /* Neumaier-compensated sum of h*area*cos(phi).  The reference's cell "area" has no
+       cos(phi) factor (src/Problema.cxx:444): x fluxes carry 1/cos(phi) and y fluxes are
+       divided by the receiving cell's cos(phi) (src/GPU/Arista_kernel.cu:683-685), so
+       the invariant of the discrete scheme is sum h*area*cos(phi). */
     double sum = 0.0, comp = 0.0;
     for (int j = 0; j < s->p.ny; j++) {
         for (int i = 0; i < s->p.nx; i++) {
-            double x = s->h[(int64_t)j * s->p.nx + i] * s->area[j];
+            double x = s->h[(int64_t)j * s->p.nx + i] * s->area[j] * s->cosphi[j];
             double t = sum + x;
             if (fabs(sum) >= fabs(x)) comp += (sum - t) + x;
             else comp += (x - t) + sum;

@@ -136,7 +136,7 @@ typedef struct orc_run_out {
 int orc_run(orc_sim *s, const orc_run_cfg *c, orc_run_out *o);
 
 /* read-only geometry/state access used by tests */
-double orc_mass(const orc_sim *s);   /* sum h*area (compensated) */
+double orc_mass(const orc_sim *s);   /* sum h*area*cos(phi) (compensated) */
 double orc_time_of(const orc_sim *s);
 
 #ifdef __cplusplus

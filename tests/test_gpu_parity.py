@@ -75,8 +75,9 @@ def test_step_replay(borders):
     ho, Ho, qxo, qyo = o.state()
     eta_o = ho - Ho - pr.hmin
     eta_g = hg - Hg - pr.hmin
-    peak = np.max(np.abs(eta_o[ho > pr.eps_h]))
-    assert rel_linf(eta_g, eta_o, peak) < 1e-4
+    wet = ho > pr.eps_h   # on dry land eta = -depth is not a model output (masked -9999)
+    peak = np.max(np.abs(eta_o[wet]))
+    assert rel_linf(eta_g[wet], eta_o[wet], peak) < 1e-4
     assert rel_linf(qxg, qxo, peak) < 1e-4
     assert rel_linf(qyg, qyo, peak) < 1e-4
 

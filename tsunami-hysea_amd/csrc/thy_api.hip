@@ -55,6 +55,20 @@ struct thy_ctx {
     double last_ms = 0.0;
     std::vector<float> g_time, g_eta, g_u, g_v;
     int nsamples = 0;
+    // schedule of the current run (kept so that thy_run_more can continue it)
+    std::vector<thy_fault> faults;
+    std::vector<double> ftimes;
+    int crop_flag = 0;
+    double crop_value = 0.0, save_interval = -1.0, next_save = INFINITY;
+    int K = 24, nsnap = 0;
+    thy_snapshot_fn on_snapshot = nullptr;
+    thy_okada_fn on_okada = nullptr;
+    thy_step_fn on_step = nullptr;
+    void *user = nullptr;
+    bool run_active = false;
+    double *steplog = nullptr;
+    int logcap = 4096;
+    int64_t logged = 0;  // iterations already delivered to on_step
     std::string err;
 };
 
@@ -81,7 +95,7 @@ struct TileShape { int tx, ty; };
 template <typename T> bool tile_supported(int tx, int ty);
 template <> bool tile_supported<float>(int tx, int ty) {
     return (tx == 64 && ty == 16) || (tx == 64 && ty == 8) || (tx == 32 && ty == 16) || (tx == 128 && ty == 8) ||
-           (tx == 64 && ty == 12);
+           (tx == 64 && ty == 12) || (tx == 32 && ty == 8) || (tx == 32 && ty == 24);
 }
 template <> bool tile_supported<double>(int tx, int ty) { return (tx == 32 && ty == 8) || (tx == 64 && ty == 4); }
 
@@ -118,6 +132,8 @@ StepArgs<T> make_args(thy_ctx *c) {
     a.gpos = c->gpos;
     a.gbuf = c->gbuf;
     a.gtime = c->gtime;
+    a.steplog = c->steplog;
+    a.logcap = c->logcap;
     return a;
 }
 
@@ -136,6 +152,8 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
         THY_TILE(32, 16)
         THY_TILE(128, 8)
         THY_TILE(64, 12)
+        THY_TILE(32, 8)
+        THY_TILE(32, 24)
     } else {
         THY_TILE(32, 8)
         THY_TILE(64, 4)
@@ -193,7 +211,7 @@ int pick_tile(thy_ctx *c) {
     int tx = c->prm.tile_x, ty = c->prm.tile_y;
     if (tx == 0 || ty == 0) {
         if (c->prec) { tx = 32; ty = 8; }
-        else { tx = 64; ty = 16; }
+        else { tx = 32; ty = 16; }
     }
     bool ok = c->prec ? tile_supported<double>(tx, ty) : tile_supported<float>(tx, ty);
     if (!ok) return fail(c, THY_ERR_INVALID, "unsupported tile shape " + std::to_string(tx) + "x" + std::to_string(ty));
@@ -443,9 +461,55 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
     }
 }
 
+// plain copy, 4 x 16 B per thread per block-chunk, chunks coalesced across the block
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n) {
+    const int64_t chunk = 4 * 256;
+    for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
+        float4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t i = base + k * 256 + threadIdx.x;
+            if (i < n) v[k] = a[i];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t i = base + k * 256 + threadIdx.x;
+            if (i < n) b[i] = v[k];
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy) {
+    thy_ctx *ctx = nullptr;
+    const int64_t n = (int64_t)(bytes / sizeof(float4));
+    if (n <= 0 || reps <= 0) return THY_ERR_INVALID;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&a, n * sizeof(float4)));
+    HIPCHK(hipMalloc(&b, n * sizeof(float4)));
+    HIPCHK(hipMemset(a, 0, n * sizeof(float4)));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int grid = (int)std::min<int64_t>((n + 1023) / 1024, 1 << 20);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return THY_OK;
+}
 
 int thy_probe(void) {
     int n = 0;
@@ -509,8 +573,14 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     HIPCHK(hipMalloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
     HIPCHK(hipMalloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
     HIPCHK(hipMalloc((void **)&ctx->mass_partial, kMassBlocks * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&ctx->steplog, (size_t)ctx->logcap * 2 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&ctx->area_d, ctx->ny * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx->area_d, g->area, ctx->ny * sizeof(double), hipMemcpyHostToDevice));
+    {
+        // mass weight per row: area*cos(phi), the invariant of the discrete scheme
+        std::vector<double> w(ctx->ny);
+        for (int j = 0; j < ctx->ny; j++) w[j] = g->area[j] * g->cosphi[j];
+        HIPCHK(hipMemcpy(ctx->area_d, w.data(), ctx->ny * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (ctx->prec) {
         std::vector<RowGeo<double>> rows;
         std::vector<FaceGeoY<double>> fys;
@@ -547,7 +617,8 @@ void thy_destroy(thy_ctx *ctx) {
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
-                    ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf};
+                    ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
+                    ctx->steplog};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -665,17 +736,103 @@ int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v, fl
     return THY_OK;
 }
 
+// deliver finished iterations [ctx->logged, upto) to the on_step callback
+static int drain_log(thy_ctx *ctx, int64_t upto) {
+    if (!ctx->on_step || upto <= ctx->logged) {
+        if (upto > ctx->logged) ctx->logged = upto;
+        return THY_OK;
+    }
+    std::vector<double> buf((size_t)ctx->logcap * 2);
+    HIPCHK(hipMemcpy(buf.data(), ctx->steplog, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t it = std::max(ctx->logged, upto - ctx->logcap); it < upto; it++) {
+        const int64_t li = it % ctx->logcap;
+        ctx->on_step(ctx->user, it + 1, buf[2 * li], buf[2 * li + 1]);
+    }
+    ctx->logged = upto;
+    return THY_OK;
+}
+
+static int run_loop(thy_ctx *ctx, thy_run_result *res) {
+    int r;
+    double total_ms = 0.0;
+    for (;;) {
+        Ctrl s;
+        double ms = 0.0;
+        if ((r = advance(ctx, ctx->K, &s, &ms)) != THY_OK) return r;
+        total_ms += ms;
+        // after a stop every finished iteration has its log entry; otherwise the last
+        // one is completed by the next launch
+        if ((r = drain_log(ctx, s.dt_valid ? s.steps : s.steps - 1)) != THY_OK) return r;
+        if (!(s.t < s.t_end) || (s.max_steps > 0 && s.steps >= s.max_steps)) {
+            res->steps = s.steps;
+            res->t_final = s.t;
+            ctx->nsamples = std::min(s.nsamples, ctx->gcap);
+            res->n_samples = ctx->nsamples;
+            break;
+        }
+        bool changed = false;
+        // src/GPU/ShallowWater.cu:125-147: apply every fault whose time has come
+        while (s.k < s.nf && s.t * s.T >= ctx->ftimes[s.k]) {
+            if (ctx->on_okada) ctx->on_okada(ctx->user, s.k);
+            if ((r = thy_apply_okada(ctx, &ctx->faults[s.k], ctx->crop_flag, ctx->crop_value, nullptr)) != THY_OK)
+                return r;
+            s.k++;
+            changed = true;
+        }
+        // src/GPU/ShallowWater.cu:404-409: snapshot
+        if (s.t >= ctx->next_save) {
+            ctx->next_save += ctx->save_interval;
+            s.next_save = ctx->next_save;
+            if (ctx->on_snapshot) ctx->on_snapshot(ctx->user, ctx->nsnap, s.t, ctx);
+            ctx->nsnap++;
+            changed = true;
+        }
+        if (!changed) return fail(ctx, THY_ERR_INVALID, "driver stalled");
+        if ((r = write_ctrl(ctx, s)) != THY_OK) return r;
+    }
+    res->n_snapshots = ctx->nsnap;
+    res->seconds = total_ms * 1e-3;
+    ctx->last_ms = total_ms;
+    // collect gauge samples
+    const int ns = ctx->nsamples, ng = ctx->ng;
+    ctx->g_time.assign(ns, 0.f);
+    ctx->g_eta.assign((size_t)ns * ng, 0.f);
+    ctx->g_u.assign((size_t)ns * ng, 0.f);
+    ctx->g_v.assign((size_t)ns * ng, 0.f);
+    if (ns > 0 && ng > 0) {
+        std::vector<float> buf((size_t)ns * ng * 3);
+        HIPCHK(hipMemcpy(buf.data(), ctx->gbuf, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ctx->g_time.data(), ctx->gtime, ns * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)ns * ng; i++) {
+            ctx->g_eta[i] = buf[3 * i];
+            ctx->g_u[i] = buf[3 * i + 1];
+            ctx->g_v[i] = buf[3 * i + 2];
+        }
+    }
+    return THY_OK;
+}
+
 int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     if (!ctx || !cfg || !res) return THY_ERR_INVALID;
     if (cfg->n_faults < 0 || (cfg->n_faults > 0 && !cfg->faults) || (cfg->n_gauges > 0 && !cfg->gauge_pos))
         return fail(ctx, THY_ERR_INVALID, "bad run configuration");
     int r;
     memset(res, 0, sizeof(*res));
-    const int K = cfg->steps_per_sync > 0 ? ((cfg->steps_per_sync + 2) / 3) * 3 : 24;
-    // fault times
+    ctx->K = cfg->steps_per_sync > 0 ? ((cfg->steps_per_sync + 2) / 3) * 3 : 24;
+    ctx->faults.assign(cfg->faults, cfg->faults + cfg->n_faults);
+    ctx->ftimes.assign(std::max(1, cfg->n_faults), 0.0);
+    for (int k = 0; k < cfg->n_faults; k++) ctx->ftimes[k] = cfg->faults[k].time;
+    ctx->crop_flag = cfg->crop_flag;
+    ctx->crop_value = cfg->crop_value;
+    ctx->save_interval = cfg->save_interval;
+    ctx->on_snapshot = cfg->on_snapshot;
+    ctx->on_okada = cfg->on_okada;
+    ctx->on_step = cfg->on_step;
+    ctx->user = cfg->user;
+    ctx->nsnap = 0;
+    ctx->logged = 0;
+    const std::vector<double> &ft = ctx->ftimes;
     if (ctx->ftime) { (void)hipFree(ctx->ftime); ctx->ftime = nullptr; }
-    std::vector<double> ft(std::max(1, cfg->n_faults), 0.0);
-    for (int k = 0; k < cfg->n_faults; k++) ft[k] = cfg->faults[k].time;
     HIPCHK(hipMalloc((void **)&ctx->ftime, ft.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->ftime, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
     // gauges
@@ -686,7 +843,7 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     ctx->gcap = 0;
     if (cfg->n_gauges > 0 && cfg->series_interval >= 0.0) {
         double nsd = cfg->series_interval > 0.0 ? cfg->t_end / cfg->series_interval + 4.0 : 1e6;
-        if (cfg->max_steps > 0) nsd = std::min(nsd, (double)cfg->max_steps + 2.0);
+        if (cfg->max_steps > 0 && cfg->series_interval <= 0.0) nsd = (double)cfg->max_steps + 2.0;
         ctx->gcap = (int)std::min(nsd, 1e6);
     }
     HIPCHK(hipMalloc((void **)&ctx->gpos, std::max(1, ctx->ng) * sizeof(int)));
@@ -712,10 +869,11 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     c.next_series = (cfg->series_interval >= 0.0 && ctx->ng > 0) ? 0.0 : INFINITY;
     c.series_dt = cfg->series_interval;
     c.gcap = ctx->gcap;
+    ctx->next_save = c.next_save;
     double dt = dt0;
     int skip = 0;
     if (cfg->n_faults > 0) {
-        auto T = ctx->prm.T;
+        const double T = ctx->prm.T;
         if (ft[0] < 2.220446049250313e-16) {
             // skipping variant (src/GPU/Timestep.cu:46-68)
             int i = 1;
@@ -732,61 +890,23 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     c.cur = ctx->cur;
     if ((r = reset_dt_slots(ctx)) != THY_OK) return r;
     if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
-    double next_save = c.next_save;
-    int nsnap = 0;
-    double total_ms = 0.0;
-    for (;;) {
-        Ctrl s;
-        double ms = 0.0;
-        if ((r = advance(ctx, K, &s, &ms)) != THY_OK) return r;
-        total_ms += ms;
-        if (!(s.t < s.t_end) || (s.max_steps > 0 && s.steps >= s.max_steps)) {
-            res->steps = s.steps;
-            res->t_final = s.t;
-            ctx->nsamples = std::min(s.nsamples, ctx->gcap);
-            res->n_samples = s.nsamples;
-            break;
-        }
-        bool changed = false;
-        // src/GPU/ShallowWater.cu:125-147: apply every fault whose time has come
-        while (s.k < s.nf && s.t * s.T >= ft[s.k]) {
-            if (cfg->on_okada) cfg->on_okada(cfg->user, s.k);
-            if ((r = thy_apply_okada(ctx, &cfg->faults[s.k], cfg->crop_flag, cfg->crop_value, nullptr)) != THY_OK)
-                return r;
-            s.k++;
-            changed = true;
-        }
-        // src/GPU/ShallowWater.cu:404-409: snapshot
-        if (s.t >= next_save) {
-            next_save += cfg->save_interval;
-            s.next_save = next_save;
-            if (cfg->on_snapshot) cfg->on_snapshot(cfg->user, nsnap, s.t, ctx);
-            nsnap++;
-            changed = true;
-        }
-        if (!changed) return fail(ctx, THY_ERR_INVALID, "driver stalled");
-        if ((r = write_ctrl(ctx, s)) != THY_OK) return r;
-    }
-    res->n_snapshots = nsnap;
-    res->seconds = total_ms * 1e-3;
-    ctx->last_ms = total_ms;
-    // collect gauge samples
-    const int ns = ctx->nsamples, ng = ctx->ng;
-    ctx->g_time.assign(ns, 0.f);
-    ctx->g_eta.assign((size_t)ns * ng, 0.f);
-    ctx->g_u.assign((size_t)ns * ng, 0.f);
-    ctx->g_v.assign((size_t)ns * ng, 0.f);
-    if (ns > 0 && ng > 0) {
-        std::vector<float> buf((size_t)ns * ng * 3);
-        HIPCHK(hipMemcpy(buf.data(), ctx->gbuf, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(ctx->g_time.data(), ctx->gtime, ns * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < (size_t)ns * ng; i++) {
-            ctx->g_eta[i] = buf[3 * i];
-            ctx->g_u[i] = buf[3 * i + 1];
-            ctx->g_v[i] = buf[3 * i + 2];
-        }
-    }
-    return THY_OK;
+    ctx->run_active = true;
+    double dti = res->dt_initial;
+    r = run_loop(ctx, res);
+    res->dt_initial = dti;
+    return r;
+}
+
+int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res) {
+    if (!ctx || !res || n_steps < 0) return THY_ERR_INVALID;
+    if (!ctx->run_active) return fail(ctx, THY_ERR_INVALID, "thy_run_more without a previous thy_run");
+    int r;
+    memset(res, 0, sizeof(*res));
+    Ctrl c;
+    if ((r = read_ctrl(ctx, &c)) != THY_OK) return r;
+    c.max_steps = c.steps + n_steps;
+    if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
+    return run_loop(ctx, res);
 }
 
 }  // extern "C"

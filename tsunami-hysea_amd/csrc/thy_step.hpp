@@ -72,6 +72,8 @@ template <typename T> struct StepArgs {
     const int *gpos;
     float *gbuf;             // [gcap][n_gauges][3]
     float *gtime;            // [gcap]
+    double *steplog;         // [logcap][2]: (dt after truncation, t) of each finished iteration
+    int logcap;
 };
 
 __device__ __forceinline__ void trunc_plain(const Ctrl &c, const double *ft, double t, double &dt, int &skip) {
@@ -145,14 +147,70 @@ __device__ __forceinline__ void near_dry(T hs, T qxo, T qyo, T &qx, T &qy, T dt,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Deep-water fast path.  When every cell of a tile and its 4-cell halo is deeper
+// than hpos and the tile is clear of the borders, sponges and the first-order band,
+// every face takes the reference's 2s branch (src/GPU/Arista_kernel.cu:545,653,
+// 1105,1214).  The 2s mass flux of a face depends only on the two adjacent cells, so
+// each cell evaluates its four mass faces itself; contraction is disabled and the
+// result is pinned in a register so that both cells of a face see the same rounded
+// value and the update stays conservative.
+// ---------------------------------------------------------------------------
+
+// x face: F = q0*alto/cos, positivity (src/GPU/Arista_kernel.cu:79-95,118-130)
+template <typename T>
+__device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T A, T dt) {
+#pragma clang fp contract(off)
+    T F = q0 * il;
+    T lim = (F > T(0)) ? T(0.25) * h0 * A : T(0.25) * h1 * A;
+    T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    if (dt * den > lim) {
+        T dt0 = lim * Num<T>::rcp(den);
+        if (dt > dt0) F *= dt0 * Num<T>::rcp(dt + Num<T>::eps);
+    }
+    asm volatile("" : "+v"(F));
+    return F;
+}
+
+// y face: F = ancho*q0*cos0, positivity with cos (src/GPU/Arista_kernel.cu:97-114,674-686)
+template <typename T>
+__device__ __forceinline__ T deep_mass_y(T q0, T h0, T h1, T len, T c0, T c1, T A0, T A1, T dt) {
+#pragma clang fp contract(off)
+    T F = len * q0 * c0;
+    T lim = (F > T(0)) ? T(0.25) * h0 * c0 * A0 : T(0.25) * h1 * c1 * A1;
+    T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    if (dt * den > lim) {
+        T dt0 = lim * Num<T>::rcp(den);
+        if (dt > dt0) F *= dt0 * Num<T>::rcp(dt + Num<T>::eps);
+    }
+    asm volatile("" : "+v"(F));
+    return F;
+}
+
+template <typename T, int TX, int TY>
+struct StepLds {
+    static constexpr int IX = TX + 8, IY = TY + 8, SX = TX + 4, SY = TY + 4;
+    static constexpr int NIN = IX * IY, NS = SX * SY, NT = TX * TY;
+    static constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1);
+    static constexpr int R0 = (4 * NIN > 4 * (NFX + NFY)) ? 4 * NIN : 4 * (NFX + NFY);
+    static constexpr int R1 = (7 * NS > 2 * NS + 3 * NT) ? 7 * NS : 2 * NS + 3 * NT;
+    static constexpr int TOTAL = R0 + R1;
+};
+
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slot) {
     constexpr int NT = 256;
-    constexpr int IX = TX + 8, IY = TY + 8, SX = TX + 4, SY = TY + 4;
-    __shared__ T s_e[IY * IX], s_D[IY * IX], s_qx[IY * IX], s_qy[IY * IX];
-    __shared__ T s_es[SY * SX];
-    __shared__ T s_am[SY * SX];
-    __shared__ T s_ax[TY * TX], s_ay[TY * TX], s_aw[TY * TX];
+    using L = StepLds<T, TX, TY>;
+    constexpr int IX = L::IX, IY = L::IY, SX = L::SX, SY = L::SY;
+    // one LDS buffer, two layouts:
+    //   region 0: input (e, D, qx, qy on tile+4); the fast path overwrites it with the
+    //             momentum face fluxes once the cell arrays hold what it needs
+    //   region 1: fast path: e*, D, sqrt(h*), u, v, qx, qy on tile+2
+    //             general path: e*, mass accumulator, momentum accumulators
+    __shared__ T lds[L::TOTAL];
+    T *s_e = lds, *s_D = lds + L::NIN, *s_qx = lds + 2 * L::NIN, *s_qy = lds + 3 * L::NIN;
+    T *r1 = lds + L::R0;
+    T *s_es = r1;
     __shared__ BlockCtl<T> bc;
     __shared__ float s_red[NT / 64];
 
@@ -184,6 +242,12 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         bc.gcap = c.gcap;
         bc.Tscale = c.T;
         if (bid == 0) {
+            // iteration c.steps has finished: record the dt chosen for the next one and the time
+            if (c.steps > 0 && a.steplog) {
+                const int64_t li = (c.steps - 1) % a.logcap;
+                a.steplog[2 * li] = dt;
+                a.steplog[2 * li + 1] = c.t;
+            }
             a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
             Ctrl o = c;
             o.skip = skip;
@@ -241,23 +305,188 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
 
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int x0 = txi * TX, y0 = tyi * TY;
+    // clear of borders (first-order band, clamped stencils), sponges and grid wrap
+    const bool interior = x0 >= 8 && x0 + TX + 8 <= nx && y0 >= 8 && y0 + TY + 8 <= ny;
 
-    // ---- stage 0: input region with clamp (or periodic wrap in x) ----
-    for (int idx = tid; idx < IX * IY; idx += NT) {
-        const int lj = idx / IX, li = idx - lj * IX;
-        int gx = x0 - 4 + li, gy = y0 - 4 + lj;
-        if (per) gx = ((gx % nx) + nx) % nx;
-        else gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
-        gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
-        const int64_t o = (int64_t)gy * P + gx;
-        s_e[idx] = ge[o];
-        s_D[idx] = a.D[o];
-        s_qx[idx] = gqx[o];
-        s_qy[idx] = gqy[o];
+    // ---- stage 0: input region (e, D, qx, qy) on tile+4 ----
+    if (interior && (sizeof(T) == 4)) {
+        constexpr int V = IX / 4;  // float4 per row
+        for (int idx = tid; idx < V * IY; idx += NT) {
+            const int lj = idx / V, lv = idx - lj * V;
+            const int64_t o = (int64_t)(y0 - 4 + lj) * P + (x0 - 4) + 4 * lv;
+            const int l = lj * IX + 4 * lv;
+            *(float4 *)&s_e[l] = *(const float4 *)&ge[o];
+            *(float4 *)&s_D[l] = *(const float4 *)&a.D[o];
+            *(float4 *)&s_qx[l] = *(const float4 *)&gqx[o];
+            *(float4 *)&s_qy[l] = *(const float4 *)&gqy[o];
+        }
+    } else {
+        for (int idx = tid; idx < IX * IY; idx += NT) {
+            const int lj = idx / IX, li = idx - lj * IX;
+            int gx = x0 - 4 + li, gy = y0 - 4 + lj;
+            if (per) gx = ((gx % nx) + nx) % nx;
+            else gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
+            gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
+            const int64_t o = (int64_t)gy * P + gx;
+            s_e[idx] = ge[o];
+            s_D[idx] = a.D[o];
+            s_qx[idx] = gqx[o];
+            s_qy[idx] = gqy[o];
+        }
     }
-    for (int idx = tid; idx < SY * SX; idx += NT) s_am[idx] = T(0);
     __syncthreads();
 
+    // deep vote over the input region
+    bool mine = true;
+    if (interior) {
+        for (int idx = tid; idx < IX * IY; idx += NT) mine = mine && (s_e[idx] + s_D[idx] > a.hpos);
+    }
+    const bool fast = __syncthreads_and(interior && mine) != 0;
+
+    float dtmin = 3.0e38f;
+    const T tnow = (T)bc.t;
+
+    if (fast) {
+        // ================= fast path =================
+        T *f_D = r1 + L::NS, *f_s = r1 + 2 * L::NS, *f_u = r1 + 3 * L::NS, *f_v = r1 + 4 * L::NS;
+        T *f_qx = r1 + 5 * L::NS, *f_qy = r1 + 6 * L::NS;
+        // h*, e* on tile+2 with the per-cell quantities of the momentum faces
+        bool deep2 = true;
+        for (int idx = tid; idx < L::NS; idx += NT) {
+            const int sj = idx / SX, si = idx - sj * SX;
+            const int gy = y0 - 2 + sj;
+            const RowGeo<T> rg = a.row[gy];
+            const FaceGeoY<T> fb = a.fy[gy], ft = a.fy[gy + 1];
+            const int io = (sj + 2) * IX + si + 2;
+            const T e = s_e[io], D = s_D[io], qx = s_qx[io], qy = s_qy[io];
+            const T h = e + D;
+            const T hl = s_e[io - 1] + s_D[io - 1], hr = s_e[io + 1] + s_D[io + 1];
+            const T hb = s_e[io - IX] + s_D[io - IX], ht = s_e[io + IX] + s_D[io + IX];
+            const T Fl = deep_mass_x(s_qx[io - 1], hl, h, rg.lenx * rg.gx, rg.A, dt);
+            const T Fr = deep_mass_x(qx, h, hr, rg.lenx * rg.gx, rg.A, dt);
+            const T Fb = deep_mass_y(s_qy[io - IX], hb, h, fb.len, fb.cos0, fb.cos1, fb.A0, fb.A1, dt);
+            const T Ft = deep_mass_y(qy, h, ht, ft.len, ft.cos0, ft.cos1, ft.A0, ft.A1, dt);
+            const T acc = (Fl - Fr) + (Fb * fb.ic1 - Ft * ft.ic0);
+            const T val = dt * rg.invA;
+            T es = e + val * acc;
+            if (!(es + D > T(0))) es = -D;
+            const T hs = es + D;
+            deep2 = deep2 && (hs > a.hpos);
+            const T ih = Num<T>::rcp(hs);
+            s_es[idx] = es;
+            f_D[idx] = D;
+            f_s[idx] = Num<T>::sqrt(hs);
+            f_u[idx] = qx * ih;
+            f_v[idx] = qy * ih;
+            f_qx[idx] = qx;
+            f_qy[idx] = qy;
+            // metrics from the state before the step (src/GPU/Metrics.cu:23-42,57-77); the
+            // update is idempotent, so a fall-back to the general path may repeat it
+            if (si >= 2 && si < TX + 2 && sj >= 2 && sj < TY + 2) {
+                const int64_t go = (int64_t)gy * P + x0 - 2 + si;
+                const T em = a.emax[go];
+                if (e > em && h > a.eps_h) a.emax[go] = e;
+                if (a.track_arrival) {
+                    const T ta = a.tarr[go];
+                    if (ta < T(0) && tabs(e - a.eini[go]) > a.dif_at) a.tarr[go] = tnow;
+                }
+            }
+        }
+        const bool fast2 = __syncthreads_and(deep2) != 0;
+        if (fast2) {
+            // momentum faces, 2s branch (src/GPU/Arista_kernel.cu:221-280, 780-843);
+            // per face: mn, pn (normal momentum to the low/high cell), ft (tangential,
+            // low -= ft, high += ft), w (wave-speed term)
+            T *fx = lds, *fyv = lds + 4 * L::NFX;
+            const T cd2s = T(0.5) * (T(1) - a.cvis);
+            for (int idx = tid; idx < L::NFX; idx += NT) {
+                const int tj = idx / (TX + 1), fi = idx - tj * (TX + 1);
+                const int gy = y0 + tj;
+                const RowGeo<T> rg = a.row[gy];
+                const int c0 = (tj + 2) * SX + fi + 1, c1 = c0 + 1;
+                const T h0 = s_es[c0] + f_D[c0], h1 = s_es[c1] + f_D[c1];
+                const T s0 = f_s[c0], s1 = f_s[c1], u0 = f_u[c0], u1 = f_u[c1];
+                const T hij = T(0.5) * (h0 + h1);
+                const T uij = (s0 * u0 + s1 * u1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
+                const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+                const T deta = deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]);
+                const T il = rg.lenx * rg.gx;
+                const bool up = uij < T(0);
+                const T uu = up ? u1 : u0;
+                const T fn = uu * (up ? f_qx[c1] : f_qx[c0]) * il;
+                const T ftg = uu * (up ? f_qy[c1] : f_qy[c0]) * il;
+                const T DES = cd2s * (rg.valx * dt) * lmax * lmax * (f_qx[c1] - f_qx[c0]) * il;
+                fx[4 * idx + 0] = fn + hij * deta * il - DES;
+                fx[4 * idx + 1] = -fn + DES;
+                fx[4 * idx + 2] = ftg;
+                fx[4 * idx + 3] = il * lmax;
+            }
+            for (int idx = tid; idx < L::NFY; idx += NT) {
+                const int fj = idx / TX, ti = idx - fj * TX;
+                const int g = y0 + fj;
+                const FaceGeoY<T> fg = a.fy[g];
+                const int c0 = (fj + 1) * SX + ti + 2, c1 = c0 + SX;
+                const T h0 = s_es[c0] + f_D[c0], h1 = s_es[c1] + f_D[c1];
+                const T s0 = f_s[c0], s1 = f_s[c1], v0 = f_v[c0], v1 = f_v[c1];
+                const T hij = T(0.5) * (h0 + h1);
+                const T uij = (s0 * v0 + s1 * v1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
+                const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+                const T deta = T(0.5) * (fg.cos0 + fg.cos1) * deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]) -
+                               uij * uij * (fg.cos1 - fg.cos0);
+                const bool up = uij < T(0);
+                const T uu = up ? v1 : v0;
+                const T fn = uu * (up ? f_qy[c1] : f_qy[c0]) * fg.len;
+                T mn, pn;
+                if (up) { pn = -fn; mn = fn * (fg.cos1 * fg.ic0); }
+                else { mn = fn; pn = -fn * fg.cos0 * fg.ic1; }
+                mn += hij * deta * fg.len * fg.ic0;
+                const T DES = cd2s * (fg.valy * dt) * lmax * lmax * (f_qy[c1] * fg.cos1 - f_qy[c0] * fg.cos0) * fg.len;
+                fyv[4 * idx + 0] = mn - DES * fg.ic0;
+                fyv[4 * idx + 1] = pn + DES * fg.ic1;
+                fyv[4 * idx + 2] = uu * (up ? f_qx[c1] : f_qx[c0]) * fg.len;
+                fyv[4 * idx + 3] = fg.len * lmax;
+            }
+            __syncthreads();
+            // cell update (src/GPU/Volumen_kernel.cu:160-221; no sponge or friction here)
+            for (int idx = tid; idx < TX * TY; idx += NT) {
+                const int tj = idx / TX, ti = idx - tj * TX;
+                const int gx = x0 + ti, gy = y0 + tj;
+                const RowGeo<T> rg = a.row[gy];
+                const int so = (tj + 2) * SX + ti + 2;
+                const int xl = tj * (TX + 1) + ti, xr = xl + 1;
+                const int yb = tj * TX + ti, yt = yb + TX;
+                const T ax = -fx[4 * xl + 1] - fx[4 * xr + 0] + fyv[4 * yb + 2] - fyv[4 * yt + 2];
+                const T ay = fx[4 * xl + 2] - fx[4 * xr + 2] - fyv[4 * yb + 1] - fyv[4 * yt + 0];
+                const T aw = (fx[4 * xl + 3] + fx[4 * xr + 3]) + (fyv[4 * yb + 3] + fyv[4 * yt + 3]);
+                const T dtc = (aw < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, aw);
+                dtmin = fminf(dtmin, (float)dtc);
+                const T val = dt * rg.invA;
+                const T qx = f_qx[so] + val * ax;
+                const T qy = f_qy[so] + val * ay;
+                const T es = s_es[so];
+                const int64_t go = (int64_t)gy * P + gx;
+                oe[go] = es;
+                oqx[go] = qx;
+                oqy[go] = qy;
+            }
+            dtmin = wave_min(dtmin);
+            if ((tid & 63) == 0) s_red[tid >> 6] = dtmin;
+            __syncthreads();
+            if (tid == 0) {
+                float m = s_red[0];
+#pragma unroll
+                for (int w = 1; w < NT / 64; w++) m = fminf(m, s_red[w]);
+                atomicMin(&a.dtmin[slot], __float_as_uint(m));
+            }
+            return;
+        }
+        // h* fell below hpos somewhere in the halo: continue on the general path with e*
+        // already in s_es (the input region is intact)
+    }
+
+    // ================= general path =================
+    T *s_am = r1 + L::NS;
+    T *s_ax = r1 + 2 * L::NS, *s_ay = s_ax + L::NT, *s_aw = s_ay + L::NT;
     Prm<T> px, py;
     px.eps_h = py.eps_h = a.eps_h;
     px.hpos = py.hpos = a.hpos;
@@ -277,91 +506,95 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         return c;
     };
 
-    // ---- stage 1: mass fluxes on tile+2, x faces (even, odd) then y faces ----
-    for (int par = 0; par < 2; par++) {
-        constexpr int NFX = (SX + 2) / 2;  // faces per row and colour (upper bound)
-        for (int idx = tid; idx < NFX * SY; idx += NT) {
-            const int sj = idx / NFX;
-            const int fi = 2 * (idx - sj * NFX) + par;
-            if (fi > SX) continue;
-            const int g = x0 - 2 + fi, gy = y0 - 2 + sj;
-            if (gy < 0 || gy >= ny) continue;
-            if (!per && (g < 0 || g > nx)) continue;
-            const int gw = per ? ((g % nx) + nx) % nx : g;
-            const int border = per ? 0 : (g == 0 ? 1 : (g == nx ? 2 : 0));
-            const bool ac = (gw < 2) || (gw > nx - 2);
-            const RowGeo<T> rg = a.row[gy];
-            const int lj = sj + 2;
-            SC<T> m = in_cell(fi, lj), c0 = in_cell(fi + 1, lj), c1 = in_cell(fi + 2, lj), q = in_cell(fi + 3, lj);
-            const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.A, rg.A, rg.valx, dt, px);
-            const bool lo = fi >= 1 && (per || g - 1 >= 0);
-            const bool hi = fi <= SX - 1 && (per || g <= nx - 1);
-            if (lo) s_am[sj * SX + fi - 1] -= F;
-            if (hi) s_am[sj * SX + fi] += (border == 1) ? -F : F;
-        }
+    if (!fast) {
+        for (int idx = tid; idx < SY * SX; idx += NT) s_am[idx] = T(0);
         __syncthreads();
-    }
-    for (int par = 0; par < 2; par++) {
-        constexpr int NFY = (SY + 2) / 2;
-        for (int idx = tid; idx < NFY * SX; idx += NT) {
-            const int k = idx / SX;
-            const int si = idx - k * SX;
-            const int fj = 2 * k + par;
-            if (fj > SY) continue;
-            const int g = y0 - 2 + fj;
-            if (g < 0 || g > ny) continue;
-            const int gx = x0 - 2 + si;
-            if (!per && (gx < 0 || gx >= nx)) continue;
-            const int border = g == 0 ? 1 : (g == ny ? 2 : 0);
-            const bool ac = (g < 2) || (g > ny - 2);
-            const FaceGeoY<T> fg = a.fy[g];
-            const int li = si + 2;
-            SC<T> m = in_cell(li, fj), c0 = in_cell(li, fj + 1), c1 = in_cell(li, fj + 2), q = in_cell(li, fj + 3);
-            m.qn = s_qy[fj * IX + li]; c0.qn = s_qy[(fj + 1) * IX + li];
-            c1.qn = s_qy[(fj + 2) * IX + li]; q.qn = s_qy[(fj + 3) * IX + li];
-            const T F = mass_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.A0, fg.A1, fg.valy, dt, py);
-            const bool lo = fj >= 1 && g - 1 >= 0;
-            const bool hi = fj <= SY - 1 && g <= ny - 1;
-            if (lo) s_am[(fj - 1) * SX + si] -= F * fg.ic0;
-            if (hi) s_am[fj * SX + si] += ((border == 1) ? -F : F) * fg.ic1;
+        // ---- stage 1: mass fluxes on tile+2, x faces (even, odd) then y faces ----
+        for (int par = 0; par < 2; par++) {
+            constexpr int NFX = (SX + 2) / 2;  // faces per row and colour (upper bound)
+            for (int idx = tid; idx < NFX * SY; idx += NT) {
+                const int sj = idx / NFX;
+                const int fi = 2 * (idx - sj * NFX) + par;
+                if (fi > SX) continue;
+                const int g = x0 - 2 + fi, gy = y0 - 2 + sj;
+                if (gy < 0 || gy >= ny) continue;
+                if (!per && (g < 0 || g > nx)) continue;
+                const int gw = per ? ((g % nx) + nx) % nx : g;
+                const int border = per ? 0 : (g == 0 ? 1 : (g == nx ? 2 : 0));
+                const bool ac = (gw < 2) || (gw > nx - 2);
+                const RowGeo<T> rg = a.row[gy];
+                const int lj = sj + 2;
+                SC<T> m = in_cell(fi, lj), c0 = in_cell(fi + 1, lj), c1 = in_cell(fi + 2, lj), q = in_cell(fi + 3, lj);
+                const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.A, rg.A, rg.valx, dt, px);
+                const bool lo = fi >= 1 && (per || g - 1 >= 0);
+                const bool hi = fi <= SX - 1 && (per || g <= nx - 1);
+                if (lo) s_am[sj * SX + fi - 1] -= F;
+                if (hi) s_am[sj * SX + fi] += (border == 1) ? -F : F;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
+        for (int par = 0; par < 2; par++) {
+            constexpr int NFY = (SY + 2) / 2;
+            for (int idx = tid; idx < NFY * SX; idx += NT) {
+                const int k = idx / SX;
+                const int si = idx - k * SX;
+                const int fj = 2 * k + par;
+                if (fj > SY) continue;
+                const int g = y0 - 2 + fj;
+                if (g < 0 || g > ny) continue;
+                const int gx = x0 - 2 + si;
+                if (!per && (gx < 0 || gx >= nx)) continue;
+                const int border = g == 0 ? 1 : (g == ny ? 2 : 0);
+                const bool ac = (g < 2) || (g > ny - 2);
+                const FaceGeoY<T> fg = a.fy[g];
+                const int li = si + 2;
+                SC<T> m = in_cell(li, fj), c0 = in_cell(li, fj + 1), c1 = in_cell(li, fj + 2), q = in_cell(li, fj + 3);
+                m.qn = s_qy[fj * IX + li]; c0.qn = s_qy[(fj + 1) * IX + li];
+                c1.qn = s_qy[(fj + 2) * IX + li]; q.qn = s_qy[(fj + 3) * IX + li];
+                const T F = mass_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.A0, fg.A1, fg.valy, dt, py);
+                const bool lo = fj >= 1 && g - 1 >= 0;
+                const bool hi = fj <= SY - 1 && g <= ny - 1;
+                if (lo) s_am[(fj - 1) * SX + si] -= F * fg.ic0;
+                if (hi) s_am[fj * SX + si] += ((border == 1) ? -F : F) * fg.ic1;
+            }
+            __syncthreads();
+        }
 
-    // ---- h* (as e*) on tile+2, with the sponge of src/GPU/Volumen_kernel.cu:96-158 ----
-    const bool any_sponge = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
-    for (int idx = tid; idx < SY * SX; idx += NT) {
-        const int sj = idx / SX, si = idx - sj * SX;
-        const int gx = x0 - 2 + si, gy = y0 - 2 + sj;
-        const int io = (sj + 2) * IX + si + 2;
-        T e = s_e[io];
-        if (gy >= 0 && gy < ny && (per || (gx >= 0 && gx < nx))) {
-            const RowGeo<T> rg = a.row[gy];
-            const T D = s_D[io];
-            e = e + dt * rg.invA * s_am[idx];
-            if (!(e + D > T(0))) e = -D;
-            if (any_sponge) {
-                const int i = per ? ((gx % nx) + nx) % nx : gx, j = gy;
-                if (i < a.sp_izq || i >= nx - a.sp_der || j < a.sp_sup || j >= ny - a.sp_inf) {
-                    const T dx = rg.dxs, dy = rg.dys;
-                    T dl = (T(a.sp_izq - i) - T(0.5)) * dx; if (dl <= T(0)) dl = T(1e30);
-                    T dr = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) * dx; if (dr <= T(0)) dr = T(1e30);
-                    T ds = (T(a.sp_sup - j) - T(0.5)) * dy; if (ds <= T(0)) ds = T(1e30);
-                    T di = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) * dy; if (di <= T(0)) di = T(1e30);
-                    T dist, ls;
-                    if (dl <= dr && dl <= ds && dl <= di) { dist = dl; ls = dx * T(a.sp_izq); }
-                    else if (dr <= dl && dr <= ds && dr <= di) { dist = dr; ls = dx * T(a.sp_der); }
-                    else if (ds <= dl && ds <= dr && ds <= di) { dist = ds; ls = dy * T(a.sp_sup); }
-                    else { dist = di; ls = dy * T(a.sp_inf); }
-                    const T r = Num<T>::div(dist, ls);
-                    const T co = T(0.9) * r * r;
-                    const T diff = (D + a.sea_level > T(0)) ? (e - a.sea_level) : (e + D);
-                    e -= co * diff;
-                    if (!(e + D >= T(0))) e = -D;
+        // ---- h* (as e*) on tile+2, with the sponge of src/GPU/Volumen_kernel.cu:96-158 ----
+        const bool any_sponge = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
+        for (int idx = tid; idx < SY * SX; idx += NT) {
+            const int sj = idx / SX, si = idx - sj * SX;
+            const int gx = x0 - 2 + si, gy = y0 - 2 + sj;
+            const int io = (sj + 2) * IX + si + 2;
+            T e = s_e[io];
+            if (gy >= 0 && gy < ny && (per || (gx >= 0 && gx < nx))) {
+                const RowGeo<T> rg = a.row[gy];
+                const T D = s_D[io];
+                e = e + dt * rg.invA * s_am[idx];
+                if (!(e + D > T(0))) e = -D;
+                if (any_sponge) {
+                    const int i = per ? ((gx % nx) + nx) % nx : gx, j = gy;
+                    if (i < a.sp_izq || i >= nx - a.sp_der || j < a.sp_sup || j >= ny - a.sp_inf) {
+                        const T dx = rg.dxs, dy = rg.dys;
+                        T dl = (T(a.sp_izq - i) - T(0.5)) * dx; if (dl <= T(0)) dl = T(1e30);
+                        T dr = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) * dx; if (dr <= T(0)) dr = T(1e30);
+                        T ds = (T(a.sp_sup - j) - T(0.5)) * dy; if (ds <= T(0)) ds = T(1e30);
+                        T di = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) * dy; if (di <= T(0)) di = T(1e30);
+                        T dist, ls;
+                        if (dl <= dr && dl <= ds && dl <= di) { dist = dl; ls = dx * T(a.sp_izq); }
+                        else if (dr <= dl && dr <= ds && dr <= di) { dist = dr; ls = dx * T(a.sp_der); }
+                        else if (ds <= dl && ds <= dr && ds <= di) { dist = ds; ls = dy * T(a.sp_sup); }
+                        else { dist = di; ls = dy * T(a.sp_inf); }
+                        const T r = Num<T>::div(dist, ls);
+                        const T co = T(0.9) * r * r;
+                        const T diff = (D + a.sea_level > T(0)) ? (e - a.sea_level) : (e + D);
+                        e -= co * diff;
+                        if (!(e + D >= T(0))) e = -D;
+                    }
                 }
             }
+            s_es[idx] = e;
         }
-        s_es[idx] = e;
     }
     for (int idx = tid; idx < TY * TX; idx += NT) {
         s_ax[idx] = T(0);
@@ -402,8 +635,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             if (fi > TX) continue;
             const int g = x0 + fi, gy = y0 + tj;
             if (gy >= ny) continue;
-            if (!per && g > nx) continue;
-            if (per && g > nx) continue;
+            if (g > nx) continue;
             const int gw = per ? (g % nx) : g;
             const int border = per ? 0 : (g == 0 ? 1 : (g == nx ? 2 : 0));
             const bool ac = (gw < 2) || (gw > nx - 2);
@@ -470,8 +702,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
     }
 
     // ---- cell update: src/GPU/Volumen_kernel.cu:160-221, metrics src/GPU/Metrics.cu ----
-    float dtmin = 3.0e38f;
-    const T tnow = (T)bc.t;
+    const bool any_sponge2 = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
     for (int idx = tid; idx < TY * TX; idx += NT) {
         const int tj = idx / TX, ti = idx - tj * TX;
         const int gx = x0 + ti, gy = y0 + tj;
@@ -499,7 +730,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         const T val = dt * rg.invA;
         T qx = qxo + val * s_ax[idx];
         T qy = qyo + val * s_ay[idx];
-        if (any_sponge) {
+        if (any_sponge2) {
             const int i = gx, j = gy;
             if (i < a.sp_izq) { const T r = (T(a.sp_izq - i) - T(0.5)) / T(a.sp_izq); qx -= T(0.9) * r * r * qx; }
             if (i >= nx - a.sp_der) { const T r = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) / T(a.sp_der); qx -= T(0.9) * r * r * qx; }

@@ -43,6 +43,7 @@ class Fault(C.Structure):
 
 SNAPFN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_void_p)
 OKADAFN = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+STEPFN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_double, C.c_double)
 
 
 class RunConfig(C.Structure):
@@ -50,7 +51,7 @@ class RunConfig(C.Structure):
                 ("n_faults", C.c_int), ("faults", C.POINTER(Fault)), ("crop_flag", C.c_int),
                 ("crop_value", C.c_double), ("n_gauges", C.c_int), ("gauge_pos", _ip),
                 ("max_steps", C.c_int64), ("steps_per_sync", C.c_int), ("on_snapshot", SNAPFN),
-                ("on_okada", OKADAFN), ("user", C.c_void_p)]
+                ("on_okada", OKADAFN), ("on_step", STEPFN), ("user", C.c_void_p)]
 
 
 class RunResult(C.Structure):
@@ -81,12 +82,14 @@ def lib():
         L.thy_initial_dt.argtypes = [C.c_void_p, _dp]
         L.thy_step.argtypes = [C.c_void_p, C.c_double, _dp]
         L.thy_run.argtypes = [C.c_void_p, C.POINTER(RunConfig), C.POINTER(RunResult)]
+        L.thy_run_more.argtypes = [C.c_void_p, C.c_int64, C.POINTER(RunResult)]
         L.thy_get_field.argtypes = [C.c_void_p, C.c_int, _fp]
         L.thy_get_gauges.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp, _fp]
         L.thy_mass.argtypes = [C.c_void_p, _dp]
         L.thy_footprint.argtypes = [C.c_int, C.c_int, C.POINTER(Params)]
         L.thy_footprint.restype = C.c_size_t
         L.thy_last_timing.argtypes = [C.c_void_p, _dp]
+        L.thy_copy_probe.argtypes = [C.c_size_t, C.c_int, _dp]
         _LIB = L
     return _LIB
 
@@ -202,7 +205,13 @@ class Solver:
         self._chk(lib().thy_mass(self.h, C.byref(v)), "mass")
         return v.value
 
-    def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None):
+    def run_more(self, n_steps):
+        """Continue the last run for n_steps more steps; returns (steps, t, gpu_seconds)."""
+        r = RunResult()
+        self._chk(lib().thy_run_more(self.h, int(n_steps), C.byref(r)), "run_more")
+        return {"steps": r.steps, "t_final": r.t_final, "seconds": r.seconds, "n_samples": r.n_samples}
+
+    def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None, on_step=None):
         pr = self.prob
         faults = sorted(pr.faults if faults is None else faults, key=lambda f: f["time"])
         farr = (Fault * max(1, len(faults)))(*[fault_of(f) for f in faults])
@@ -215,8 +224,10 @@ class Solver:
         c.max_steps, c.steps_per_sync = int(max_steps), int(steps_per_sync)
         snap = SNAPFN(lambda u, i, t, ctx: on_snapshot(i, t)) if on_snapshot else SNAPFN()
         oka = OKADAFN(lambda u, k: on_okada(k)) if on_okada else OKADAFN()
-        c.on_snapshot, c.on_okada = snap, oka
+        stp = STEPFN(lambda u, it, dt, t: on_step(it, dt, t)) if on_step else STEPFN()
+        c.on_snapshot, c.on_okada, c.on_step = snap, oka, stp
         r = RunResult()
+        self._cb = (snap, oka, stp, farr, gpos)  # callbacks must outlive a later run_more
         self._chk(lib().thy_run(self.h, C.byref(c), C.byref(r)), "run")
         ng = len(gpos)
         ns = r.n_samples
@@ -228,6 +239,15 @@ class Solver:
                 "g_eta": bufs[1][:ns * ng].reshape(ns, ng), "g_u": bufs[2][:ns * ng].reshape(ns, ng),
                 "g_v": bufs[3][:ns * ng].reshape(ns, ng), "g_eta_min": bufs[4][:ng], "g_eta_max": bufs[5][:ng],
                 "g_bathy": bufs[6][:ng]}
+
+
+def copy_probe(nbytes, reps=10):
+    """Time a plain device copy of nbytes; returns (ms per copy, GB/s of read+write)."""
+    ms = C.c_double()
+    rc = lib().thy_copy_probe(int(nbytes), int(reps), C.byref(ms))
+    if rc != 0:
+        raise ThyError(f"copy probe: {STATUS.get(rc, rc)}")
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def footprint(nx, ny, precision=0, track_arrival=True):
