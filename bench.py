@@ -8,7 +8,7 @@ seamounts, coastline along the east and north edges, four 100x50 km subfaults wi
 region starts.  Prints one JSON line (rank 0, N = 1):
 
   value        Gcell-steps/s over exactly --steps steps after --warmup steps
-  roofline     dominant kernel (k_step): algorithmic bytes per launch / measured
+  roofline     dominant kernel (the step kernel): algorithmic bytes per launch / measured
                launch time (HIP events on the library's stream) vs 8 TB/s
   cpu_baseline FP64 restatement of the reference scheme (the oracle) timed on the
                host cores for a bounded sample of the same workload
@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "tsunami-hysea_amd"))
 METRIC = "Gcell-steps/s at 12M cells; batched scenarios/hr + p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
 B_ALG = {False: 36.0, True: 44.0}  # bytes per cell-step (arrival tracking off / on)
+KERNEL_NAME = {"sweep": "k_sweep", "tile": "k_step"}
 
 
 def log(*a):
@@ -56,7 +57,9 @@ def main():
     ap.add_argument("--nx", type=int, default=4000)
     ap.add_argument("--ny", type=int, default=3000)
     ap.add_argument("--no-arrival", action="store_true", help="disable arrival-time tracking")
-    ap.add_argument("--tile", default="0x0")
+    ap.add_argument("--kernel", default="sweep", choices=["sweep", "tile"], help="step kernel")
+    ap.add_argument("--strip-rows", type=int, default=0, help="row-sweep kernel: rows per strip (0: default)")
+    ap.add_argument("--tile", default="0x0", help="2D-tile kernel: tile shape (0x0: default)")
     ap.add_argument("--sync-every", type=int, default=48, help="steps per captured graph replay")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -87,7 +90,8 @@ def main():
         copy = {"bytes": args.copy_probe_mib << 20, "ms": ms, "GBs_read_plus_write": gbs}
         log(f"copy probe {args.copy_probe_mib} MiB: {ms:.3f} ms, {gbs:.0f} GB/s")
 
-    solver = native.Solver(prob, precision=0, track_arrival=track, tile=tile)
+    solver = native.Solver(prob, precision=0, track_arrival=track, tile=tile, kernel=args.kernel,
+                           strip_rows=args.strip_rows)
     log("warmup", args.warmup, "steps (includes Okada at t=0)")
     w = solver.run(max_steps=args.warmup, steps_per_sync=args.sync_every)
     log("warmup done: t =", w["t_final"] * prob.T, "s")
@@ -104,7 +108,7 @@ def main():
     log(f"timed {args.steps} steps: wall {wall*1e3:.2f} ms, gpu {gpu_s*1e3:.2f} ms, {value:.1f} Gcell-steps/s")
 
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "k_step_12M_traffic.json")
+    tp = os.path.join(ROOT, "profiles", f"{KERNEL_NAME[args.kernel]}_12M_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -126,10 +130,12 @@ def main():
         "config": {"workload": f"{prob.nx}x{prob.ny} lon/lat grid ({ncell/1e6:.1f}M cells, 2 arc-min), "
                                "4 Okada subfaults, 64 gauges, open borders",
                    "cells": ncell, "track_arrival": track, "steps_per_sync": args.sync_every,
+                   "step_kernel": args.kernel,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_cell_step": B_ALG[track]},
+                     "kernel": KERNEL_NAME[args.kernel], "kernel_ms": kern_ms,
+                     "bytes_per_cell_step": B_ALG[track]},
         "cpu_baseline": cpu,
     }
     if copy:

@@ -60,7 +60,9 @@ typedef struct thy_params {
     double L, H, Q, T; /* scales */
     int precision;     /* 0: FP32 state and fluxes; 1: FP64 everywhere */
     int track_arrival; /* 1: arrival times tracked (reference behaviour) */
-    int tile_x, tile_y;/* 0: autotuned/default tile shape */
+    int tile_x, tile_y;/* 2D-tile kernel: tile shape, 0: default */
+    int kernel;        /* step kernel: 0 row sweep (default), 1 2D LDS tiles */
+    int strip_rows;    /* row-sweep kernel: rows per strip, 0: default */
 } thy_params;
 
 /* One Okada fault, fields as a parameter-file fault line (src/Problema.cxx:128-148). */

@@ -22,13 +22,21 @@ def _oracle(pr, **kw):
     return O.OracleSim(pr, **kw)
 
 
+# step kernels under test: the row sweep with its default strip height and with short
+# strips (many strips, ragged last strip), and the 2D LDS-tile kernel
+KERNELS = [pytest.param(dict(kernel="sweep"), id="sweep"),
+           pytest.param(dict(kernel="sweep", strip_rows=5), id="sweep5"),
+           pytest.param(dict(kernel="tile"), id="tile")]
+
+
 def test_probe_gfx950():
     assert _native().probe() == 0
 
 
-def test_lake_at_rest():
+@pytest.mark.parametrize("kw", KERNELS)
+def test_lake_at_rest(kw):
     pr = make_problem(fault=False)
-    s = _native().Solver(pr)
+    s = _native().Solver(pr, **kw)
     dt = s.initial_dt()
     h0, H0, qx0, qy0 = s.state()
     for _ in range(20):
@@ -57,11 +65,8 @@ def test_okada_matches_oracle():
     assert rel_linf(dg, do, scale) < 1e-10
 
 
-@pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, -1, -1, -1), (1, -1, 2, 2)])
-def test_step_replay(borders):
-    """Same dt sequence on both sides (the oracle's), compare after N steps."""
-    pr = make_problem(nx=70, ny=52, borders=borders)
-    s = _native().Solver(pr)
+def _replay(pr, kw, steps=40):
+    s = _native().Solver(pr, **kw)
     o = _oracle(pr)
     f = pr.faults[0]
     s.okada(f)
@@ -77,14 +82,41 @@ def test_step_replay(borders):
     eta_g = hg - Hg - pr.hmin
     wet = ho > pr.eps_h   # on dry land eta = -depth is not a model output (masked -9999)
     peak = np.max(np.abs(eta_o[wet]))
-    assert rel_linf(eta_g[wet], eta_o[wet], peak) < 1e-4
-    assert rel_linf(qxg, qxo, peak) < 1e-4
-    assert rel_linf(qyg, qyo, peak) < 1e-4
+    return (rel_linf(eta_g[wet], eta_o[wet], peak), rel_linf(qxg, qxo, peak), rel_linf(qyg, qyo, peak))
 
 
-def test_mass_walls():
+@pytest.mark.parametrize("kw", KERNELS)
+@pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, -1, -1, -1), (1, -1, 2, 2)])
+def test_step_replay(borders, kw):
+    """Same dt sequence on both sides (the oracle's), compare after N steps; FP32 state
+    within 1e-4 of the peak |eta|."""
+    pr = make_problem(nx=70, ny=52, borders=borders)
+    errs = _replay(pr, kw)
+    assert max(errs) < 1e-4, errs
+
+
+@pytest.mark.parametrize("kw", KERNELS)
+@pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, 1, 2, 2)])
+def test_step_replay_multi_strip(borders, kw):
+    """A grid spanning several 56-column sweep segments (ragged last one) and several
+    row strips, with sponges on the open borders."""
+    pr = make_problem(nx=131, ny=97, borders=borders, seamounts=5, seed=11)
+    errs = _replay(pr, kw, steps=30)
+    assert max(errs) < 1e-4, errs
+
+
+@pytest.mark.parametrize("kw", KERNELS)
+def test_step_replay_fp64(kw):
+    """FP64-everywhere path: the oracle's arithmetic to round-off."""
+    pr = make_problem(nx=70, ny=52)
+    errs = _replay(pr, dict(kw, precision=1))
+    assert max(errs) < 1e-9, errs
+
+
+@pytest.mark.parametrize("kw", KERNELS)
+def test_mass_walls(kw):
     pr = make_problem(nx=64, ny=48, borders=(-1, -1, -1, -1))
-    s = _native().Solver(pr)
+    s = _native().Solver(pr, **kw)
     s.okada(pr.faults[0])
     m0 = s.mass()
     dt = s.initial_dt()
@@ -93,14 +125,15 @@ def test_mass_walls():
     assert abs(s.mass() - m0) / m0 < 1e-6
 
 
-def test_full_run_small_matches_oracle():
+@pytest.mark.parametrize("kw", KERNELS)
+def test_full_run_small_matches_oracle(kw):
     pr = make_problem(nx=96, ny=80, t_sim=900.0)
     from thysea import scenario as S
     glon, glat = S.gauges_lattice(pr.lon, pr.lat, 3)
     from thysea import problem as P
     pr.gauge_pos = np.array([P.nearest_cell(pr.lon, pr.lat, a, b) for a, b in zip(glon, glat)], np.int32)
     pr.series_interval = 30.0 / pr.T
-    s = _native().Solver(pr)
+    s = _native().Solver(pr, **kw)
     rg = s.run()
     o = _oracle(pr)
     ro = o.run()

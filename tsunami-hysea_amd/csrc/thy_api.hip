@@ -15,12 +15,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/thysea.h"
 #include "thy_aux.hpp"
+#include "thy_sweep.hpp"
 
 using namespace thy;
 
@@ -50,6 +52,9 @@ struct thy_ctx {
     int slot = 0;
     int cur = 0;                 // host mirror of ctrl.cur after a sync
     int tx = 64, ty = 16;
+    int chunk_rows = 1;          // tile (strip) rows per XCD chunk of the step kernel's order
+    int kernel = 0;              // 0 row sweep, 1 2D tiles
+    int srows = 64;              // row sweep: rows per strip
     hipGraphExec_t gexec = nullptr;
     int gk = 0;
     double last_ms = 0.0;
@@ -106,10 +111,17 @@ StepArgs<T> make_args(thy_ctx *c) {
     const thy_params &p = c->prm;
     a.nx = c->nx; a.ny = c->ny; a.pitch = c->pitch;
     a.periodic = (std::fabs(p.borde_izq - 2.0) < 2.220446049250313e-16 && std::fabs(p.borde_der - 2.0) < 2.220446049250313e-16) ? 1 : 0;
-    a.tiles_x = (c->nx + c->tx - 1) / c->tx;
-    a.tiles_y = (c->ny + c->ty - 1) / c->ty;
+    if (c->kernel == 0) {
+        const int nseg = (c->nx + SW_OUT - 1) / SW_OUT;
+        a.tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES;
+        a.tiles_y = (c->ny + c->srows - 1) / c->srows;
+        a.srows = c->srows;
+    } else {
+        a.tiles_x = (c->nx + c->tx - 1) / c->tx;
+        a.tiles_y = (c->ny + c->ty - 1) / c->ty;
+    }
     a.n_tiles = a.tiles_x * a.tiles_y;
-    a.q_per_xcd = (a.n_tiles + 7) / 8;
+    a.chunk = a.tiles_x * c->chunk_rows;
     a.sp_sup = p.sponge_sup; a.sp_inf = p.sponge_inf; a.sp_izq = p.sponge_izq; a.sp_der = p.sponge_der;
     a.track_arrival = p.track_arrival;
     a.n_gauges = c->ng;
@@ -140,7 +152,12 @@ StepArgs<T> make_args(thy_ctx *c) {
 template <typename T>
 hipError_t launch_step_t(thy_ctx *c, int slot) {
     StepArgs<T> a = make_args<T>(c);
-    dim3 grid(a.q_per_xcd * 8), block(256);
+    const int n_chunks = (a.tiles_y + c->chunk_rows - 1) / c->chunk_rows;
+    dim3 grid(((n_chunks + 7) / 8) * 8 * a.chunk), block(256);
+    if (c->kernel == 0) {
+        hipLaunchKernelGGL((k_sweep<T>), grid, dim3(64 * SW_WAVES), 0, c->stream, a, slot);
+        return hipGetLastError();
+    }
 #define THY_TILE(TX_, TY_)                                                                            \
     if (c->tx == TX_ && c->ty == TY_) {                                                              \
         hipLaunchKernelGGL((k_step<T, TX_, TY_>), grid, block, 0, c->stream, a, slot);               \
@@ -217,6 +234,17 @@ int pick_tile(thy_ctx *c) {
     if (!ok) return fail(c, THY_ERR_INVALID, "unsupported tile shape " + std::to_string(tx) + "x" + std::to_string(ty));
     c->tx = tx;
     c->ty = ty;
+    if (c->prm.kernel != 0 && c->prm.kernel != 1)
+        return fail(c, THY_ERR_INVALID, "unknown step kernel " + std::to_string(c->prm.kernel));
+    c->kernel = c->prm.kernel;
+    if (c->prm.strip_rows < 0 || c->prm.strip_rows > 65536)
+        return fail(c, THY_ERR_INVALID, "strip_rows out of range: " + std::to_string(c->prm.strip_rows));
+    if (c->prm.strip_rows > 0) c->srows = c->prm.strip_rows;
+    // THYSEA_CHUNK_ROWS overrides the number of tile rows dealt to one XCD at a time
+    if (const char *v = std::getenv("THYSEA_CHUNK_ROWS")) {
+        const int r = std::atoi(v);
+        if (r >= 1 && r <= 4096) c->chunk_rows = r;
+    }
     return THY_OK;
 }
 

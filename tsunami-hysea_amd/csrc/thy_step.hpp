@@ -54,7 +54,8 @@ template <typename T> struct FaceGeoY {
 
 template <typename T> struct StepArgs {
     int nx, ny, pitch, periodic;
-    int tiles_x, tiles_y, n_tiles, q_per_xcd;
+    int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
+    int srows;                             // row-sweep kernel: rows per strip
     int sp_sup, sp_inf, sp_izq, sp_der;
     int track_arrival, n_gauges;
     T bsup, binf, bizq, bder;
@@ -111,6 +112,86 @@ template <typename T> struct BlockCtl {
     int run, sample, slot_sample, cur, gcap;
 };
 
+// Start of a step launch (thread 0 of every workgroup): read the control block of this
+// launch, finalise dt (truncation at fault times, src/GPU/Timestep.cu:35-68) and the
+// stop predicate; the writer (workgroup 0) also logs the finished iteration, resets
+// the dt slot of the next launch and publishes the next control block.
+template <typename T>
+__device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot, const bool writer, BlockCtl<T> &bc) {
+    const Ctrl c = a.ctrl[slot];
+    double dt;
+    int skip = c.skip;
+    if (c.dt_valid) {
+        dt = c.dt;
+    } else {
+        dt = (double)__uint_as_float(a.dtmin[(slot + 2) % 3]);
+        if (!skip) trunc_plain(c, a.ftime, c.t, dt, skip);
+        else trunc_skipping(c, a.ftime, c.t, dt, skip);
+    }
+    const bool stop = must_stop(c, a.ftime);
+    const bool sample = !stop && (c.t >= c.next_series);
+    bc.t = c.t;
+    bc.dt = dt;
+    bc.run = stop ? 0 : 1;
+    bc.sample = sample ? 1 : 0;
+    bc.slot_sample = c.nsamples;
+    bc.cur = c.cur;
+    bc.gcap = c.gcap;
+    bc.Tscale = c.T;
+    if (writer) {
+        // iteration c.steps has finished: record the dt chosen for the next one and the time
+        if (c.steps > 0 && a.steplog) {
+            const int64_t li = (c.steps - 1) % a.logcap;
+            a.steplog[2 * li] = dt;
+            a.steplog[2 * li + 1] = c.t;
+        }
+        a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
+        Ctrl o = c;
+        o.skip = skip;
+        if (stop) {
+            o.dt = dt;
+            o.dt_valid = 1;
+        } else {
+            o.t = c.t + dt;
+            o.dt_valid = 0;
+            o.steps = c.steps + 1;
+            o.cur = c.cur ^ 1;
+            if (sample) {
+                o.next_series = c.next_series + c.series_dt;
+                o.nsamples = c.nsamples + 1;
+            }
+        }
+        a.ctrl[(slot + 1) % 3] = o;
+    }
+}
+
+// gauge gather from the state at time t (src/GPU/Volumen_kernel.cu:223-254), run by
+// workgroup 0 of a step launch that takes a sample
+template <typename T>
+__device__ __forceinline__ void gather_gauges(const StepArgs<T> &a, const BlockCtl<T> &bc, const T *ge,
+                                              const T *gqx, const T *gqy, const int tid, const int nthreads) {
+    if (!bc.sample || bc.slot_sample >= bc.gcap) return;
+    const int nx = a.nx, P = a.pitch;
+    for (int g = tid; g < a.n_gauges; g += nthreads) {
+        const int p = a.gpos[g];
+        float ev = -9999.f, uv = -9999.f, vv = -9999.f;
+        if (p >= 0) {
+            const int64_t o = (int64_t)(p / nx) * P + (p % nx);
+            T e = ge[o], D = a.D[o];
+            T h = e + D;
+            T f = (h < a.eps_h) ? desing(h, a.eps_h) : Num<T>::rcp(h);
+            ev = (float)(e * a.Href);
+            uv = (float)(f * gqx[o] * a.Q_over_H);
+            vv = (float)(f * gqy[o] * a.Q_over_H);
+        }
+        float *dst = a.gbuf + ((int64_t)bc.slot_sample * a.n_gauges + g) * 3;
+        dst[0] = ev;
+        dst[1] = uv;
+        dst[2] = vv;
+    }
+    if (tid == 0) a.gtime[bc.slot_sample] = (float)(bc.t * bc.Tscale);
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
@@ -144,6 +225,52 @@ __device__ __forceinline__ void near_dry(T hs, T qxo, T qyo, T &qx, T &qy, T dt,
         T g = Num<T>::div(hs, T(1) + c1);
         qx = g * u1x;
         qy = g * u1y;
+    }
+}
+
+// sponge of the eta update on cell (i, j) (src/GPU/Volumen_kernel.cu:96-158)
+template <typename T>
+__device__ __forceinline__ T sponge_eta(const StepArgs<T> &a, T e, const T D, const int i, const int j, const T dx,
+                                        const T dy) {
+    const int nx = a.nx, ny = a.ny;
+    if (i < a.sp_izq || i >= nx - a.sp_der || j < a.sp_sup || j >= ny - a.sp_inf) {
+        T dl = (T(a.sp_izq - i) - T(0.5)) * dx; if (dl <= T(0)) dl = T(1e30);
+        T dr = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) * dx; if (dr <= T(0)) dr = T(1e30);
+        T ds = (T(a.sp_sup - j) - T(0.5)) * dy; if (ds <= T(0)) ds = T(1e30);
+        T di = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) * dy; if (di <= T(0)) di = T(1e30);
+        T dist, ls;
+        if (dl <= dr && dl <= ds && dl <= di) { dist = dl; ls = dx * T(a.sp_izq); }
+        else if (dr <= dl && dr <= ds && dr <= di) { dist = dr; ls = dx * T(a.sp_der); }
+        else if (ds <= dl && ds <= dr && ds <= di) { dist = ds; ls = dy * T(a.sp_sup); }
+        else { dist = di; ls = dy * T(a.sp_inf); }
+        const T r = Num<T>::div(dist, ls);
+        const T co = T(0.9) * r * r;
+        const T diff = (D + a.sea_level > T(0)) ? (e - a.sea_level) : (e + D);
+        e -= co * diff;
+        if (!(e + D >= T(0))) e = -D;
+    }
+    return e;
+}
+
+// sponge of the discharge update on cell (i, j) (src/GPU/Volumen_kernel.cu:160-221)
+template <typename T>
+__device__ __forceinline__ void sponge_q(const StepArgs<T> &a, const int i, const int j, T &qx, T &qy) {
+    const int nx = a.nx, ny = a.ny;
+    if (i < a.sp_izq) { const T r = (T(a.sp_izq - i) - T(0.5)) / T(a.sp_izq); qx -= T(0.9) * r * r * qx; }
+    if (i >= nx - a.sp_der) { const T r = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) / T(a.sp_der); qx -= T(0.9) * r * r * qx; }
+    if (j < a.sp_sup) { const T r = (T(a.sp_sup - j) - T(0.5)) / T(a.sp_sup); qy -= T(0.9) * r * r * qy; }
+    if (j >= ny - a.sp_inf) { const T r = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) / T(a.sp_inf); qy -= T(0.9) * r * r * qy; }
+}
+
+// max-height and arrival-time update from the state before the step
+// (src/GPU/Metrics.cu:23-42,57-77); idempotent within a step
+template <typename T>
+__device__ __forceinline__ void metrics_cell(const StepArgs<T> &a, const int64_t go, const T eo, const T ho, const T tnow) {
+    const T em = a.emax[go];
+    if (eo > em && ho > a.eps_h) a.emax[go] = eo;
+    if (a.track_arrival) {
+        const T ta = a.tarr[go];
+        if (ta < T(0) && tabs(eo - a.eini[go]) > a.dif_at) a.tarr[go] = tnow;
     }
 }
 
@@ -215,58 +342,17 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
     __shared__ float s_red[NT / 64];
 
     const int tid = threadIdx.x;
-    // XCD-aware tile order: consecutive workgroups land on different XCDs, so give
-    // each XCD a contiguous band of tiles (its own L2 then holds the shared halos).
+    // XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin, so
+    // bid & 7 is the XCD and bid >> 3 the workgroup's place in that XCD's sequence.
+    // The grid is cut into chunks of whole tile rows, dealt to the XCDs in turn: each
+    // XCD walks its chunks in row order (its L2 holds the halos shared along a row),
+    // and coastal rows, whose tiles take the slower general path, are spread over all
+    // XCDs instead of landing on one.
     const int bid = blockIdx.x;
-    const int tile = (bid & 7) * a.q_per_xcd + (bid >> 3);
+    const int loc = bid >> 3, ch = loc / a.chunk;
+    const int tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
 
-    if (tid == 0) {
-        const Ctrl c = a.ctrl[slot];
-        double dt;
-        int skip = c.skip;
-        if (c.dt_valid) {
-            dt = c.dt;
-        } else {
-            dt = (double)__uint_as_float(a.dtmin[(slot + 2) % 3]);
-            if (!skip) trunc_plain(c, a.ftime, c.t, dt, skip);
-            else trunc_skipping(c, a.ftime, c.t, dt, skip);
-        }
-        const bool stop = must_stop(c, a.ftime);
-        const bool sample = !stop && (c.t >= c.next_series);
-        bc.t = c.t;
-        bc.dt = dt;
-        bc.run = stop ? 0 : 1;
-        bc.sample = sample ? 1 : 0;
-        bc.slot_sample = c.nsamples;
-        bc.cur = c.cur;
-        bc.gcap = c.gcap;
-        bc.Tscale = c.T;
-        if (bid == 0) {
-            // iteration c.steps has finished: record the dt chosen for the next one and the time
-            if (c.steps > 0 && a.steplog) {
-                const int64_t li = (c.steps - 1) % a.logcap;
-                a.steplog[2 * li] = dt;
-                a.steplog[2 * li + 1] = c.t;
-            }
-            a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
-            Ctrl o = c;
-            o.skip = skip;
-            if (stop) {
-                o.dt = dt;
-                o.dt_valid = 1;
-            } else {
-                o.t = c.t + dt;
-                o.dt_valid = 0;
-                o.steps = c.steps + 1;
-                o.cur = c.cur ^ 1;
-                if (sample) {
-                    o.next_series = c.next_series + c.series_dt;
-                    o.nsamples = c.nsamples + 1;
-                }
-            }
-            a.ctrl[(slot + 1) % 3] = o;
-        }
-    }
+    if (tid == 0) begin_step(a, slot, bid == 0, bc);
     __syncthreads();
     if (!bc.run || tile >= a.n_tiles) return;
 
@@ -281,27 +367,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
     const int nx = a.nx, ny = a.ny, P = a.pitch;
     const bool per = a.periodic != 0;
 
-    // gauge gather from the state at time t (src/GPU/Volumen_kernel.cu:223-254)
-    if (bid == 0 && bc.sample && bc.slot_sample < bc.gcap) {
-        for (int g = tid; g < a.n_gauges; g += NT) {
-            const int p = a.gpos[g];
-            float ev = -9999.f, uv = -9999.f, vv = -9999.f;
-            if (p >= 0) {
-                const int64_t o = (int64_t)(p / nx) * P + (p % nx);
-                T e = ge[o], D = a.D[o];
-                T h = e + D;
-                T f = (h < a.eps_h) ? desing(h, a.eps_h) : Num<T>::rcp(h);
-                ev = (float)(e * a.Href);
-                uv = (float)(f * gqx[o] * a.Q_over_H);
-                vv = (float)(f * gqy[o] * a.Q_over_H);
-            }
-            float *dst = a.gbuf + ((int64_t)bc.slot_sample * a.n_gauges + g) * 3;
-            dst[0] = ev;
-            dst[1] = uv;
-            dst[2] = vv;
-        }
-        if (tid == 0) a.gtime[bc.slot_sample] = (float)(bc.t * bc.Tscale);
-    }
+    if (bid == 0) gather_gauges(a, bc, ge, gqx, gqy, tid, NT);
 
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int x0 = txi * TX, y0 = tyi * TY;
@@ -383,13 +449,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             // metrics from the state before the step (src/GPU/Metrics.cu:23-42,57-77); the
             // update is idempotent, so a fall-back to the general path may repeat it
             if (si >= 2 && si < TX + 2 && sj >= 2 && sj < TY + 2) {
-                const int64_t go = (int64_t)gy * P + x0 - 2 + si;
-                const T em = a.emax[go];
-                if (e > em && h > a.eps_h) a.emax[go] = e;
-                if (a.track_arrival) {
-                    const T ta = a.tarr[go];
-                    if (ta < T(0) && tabs(e - a.eini[go]) > a.dif_at) a.tarr[go] = tnow;
-                }
+                metrics_cell(a, (int64_t)gy * P + x0 - 2 + si, e, h, tnow);
             }
         }
         const bool fast2 = __syncthreads_and(deep2) != 0;
@@ -572,26 +632,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T D = s_D[io];
                 e = e + dt * rg.invA * s_am[idx];
                 if (!(e + D > T(0))) e = -D;
-                if (any_sponge) {
-                    const int i = per ? ((gx % nx) + nx) % nx : gx, j = gy;
-                    if (i < a.sp_izq || i >= nx - a.sp_der || j < a.sp_sup || j >= ny - a.sp_inf) {
-                        const T dx = rg.dxs, dy = rg.dys;
-                        T dl = (T(a.sp_izq - i) - T(0.5)) * dx; if (dl <= T(0)) dl = T(1e30);
-                        T dr = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) * dx; if (dr <= T(0)) dr = T(1e30);
-                        T ds = (T(a.sp_sup - j) - T(0.5)) * dy; if (ds <= T(0)) ds = T(1e30);
-                        T di = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) * dy; if (di <= T(0)) di = T(1e30);
-                        T dist, ls;
-                        if (dl <= dr && dl <= ds && dl <= di) { dist = dl; ls = dx * T(a.sp_izq); }
-                        else if (dr <= dl && dr <= ds && dr <= di) { dist = dr; ls = dx * T(a.sp_der); }
-                        else if (ds <= dl && ds <= dr && ds <= di) { dist = ds; ls = dy * T(a.sp_sup); }
-                        else { dist = di; ls = dy * T(a.sp_inf); }
-                        const T r = Num<T>::div(dist, ls);
-                        const T co = T(0.9) * r * r;
-                        const T diff = (D + a.sea_level > T(0)) ? (e - a.sea_level) : (e + D);
-                        e -= co * diff;
-                        if (!(e + D >= T(0))) e = -D;
-                    }
-                }
+                if (any_sponge) e = sponge_eta(a, e, D, per ? ((gx % nx) + nx) % nx : gx, gy, rg.dxs, rg.dys);
             }
             s_es[idx] = e;
         }
@@ -714,29 +755,14 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         const T eo = s_e[io];
         const T qxo = s_qx[io], qyo = s_qy[io];
         const int64_t go = (int64_t)gy * P + gx;
-        // metrics from the state before the step
-        {
-            const T ho = eo + D;
-            const T em = a.emax[go];
-            if (eo > em && ho > a.eps_h) a.emax[go] = eo;
-            if (a.track_arrival) {
-                const T ta = a.tarr[go];
-                if (ta < T(0) && tabs(eo - a.eini[go]) > a.dif_at) a.tarr[go] = tnow;
-            }
-        }
+        metrics_cell(a, go, eo, eo + D, tnow);  // from the state before the step
         const T aw = s_aw[idx];
         const T dtc = (aw < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, aw);
         dtmin = fminf(dtmin, (float)dtc);
         const T val = dt * rg.invA;
         T qx = qxo + val * s_ax[idx];
         T qy = qyo + val * s_ay[idx];
-        if (any_sponge2) {
-            const int i = gx, j = gy;
-            if (i < a.sp_izq) { const T r = (T(a.sp_izq - i) - T(0.5)) / T(a.sp_izq); qx -= T(0.9) * r * r * qx; }
-            if (i >= nx - a.sp_der) { const T r = (T(i - (nx - 1 - a.sp_der)) - T(0.5)) / T(a.sp_der); qx -= T(0.9) * r * r * qx; }
-            if (j < a.sp_sup) { const T r = (T(a.sp_sup - j) - T(0.5)) / T(a.sp_sup); qy -= T(0.9) * r * r * qy; }
-            if (j >= ny - a.sp_inf) { const T r = (T(j - (ny - 1 - a.sp_inf)) - T(0.5)) / T(a.sp_inf); qy -= T(0.9) * r * r * qy; }
-        }
+        if (any_sponge2) sponge_q(a, gx, gy, qx, qy);
         const T es = s_es[so];
         const T hs = es + D;
         if (hs <= a.hpos) near_dry(hs, qxo, qyo, qx, qy, dt, a.mf0, a.vmax, a.eps_h);

@@ -1,0 +1,429 @@
+// thy_sweep.hpp -- row-sweep step kernel of the Tsunami-HySEA scheme for gfx950.
+//
+// Same step as k_step (src/GPU/Timestep.cu:70-120 plus the metric update and the
+// gauge gather), organised for the CDNA execution model instead of 2D LDS tiles:
+//   * one wavefront owns 64 adjacent columns (56 output columns plus a 4-column halo
+//     on each side, the reach of the 2s+WAF stencil through the two passes) and walks
+//     a strip of rows from top to bottom;
+//   * each row flows through a register pipeline: load row r, mass x-faces of row r,
+//     mass y-face between rows r-2 and r-1, h* of row r-2, momentum x-faces of row
+//     r-2, momentum y-face between rows r-4 and r-3, cell update of row r-4;
+//   * x-neighbours come from adjacent lanes (cross-lane moves), y-neighbours from the
+//     previous rows kept in registers, so every face is evaluated exactly once and no
+//     LDS, barrier or halo re-load is needed inside a strip;
+//   * the row geometry is uniform across the wavefront (scalar loads) and each row is
+//     fetched one iteration before it is used;
+//   * each stage votes across the wavefront: when every lane's stencil is deeper than
+//     hpos and clear of the first-order band, the stage uses the deep-water 2s forms
+//     below; otherwise it uses the general per-face functions of k_step
+//     (thy_flux.hpp), with the same border, first-order band and clamped-stencil
+//     conventions.
+#pragma once
+#include "thy_step.hpp"
+
+namespace thy {
+
+constexpr int SW_HALO = 4;
+constexpr int SW_OUT = 64 - 2 * SW_HALO;  // output columns per wavefront
+constexpr int SW_WAVES = 4;               // wavefronts per workgroup (side by side in x)
+
+// Cross-lane moves: lane l receives the value of lane l-1 / l-2 / l+1 (a lane without
+// a source keeps its own value; lanes 0, 1 and 63 lie outside every output cell's
+// stencil).  These go through ds_bpermute on the LDS pipe, which leaves the VALU (the
+// unit this kernel saturates) to the flux arithmetic; DPP row moves were measured
+// slower here for that reason.
+template <typename T> __device__ __forceinline__ T from_left(T v) { return __shfl_up(v, 1, 64); }
+template <typename T> __device__ __forceinline__ T from_left2(T v) { return __shfl_up(v, 2, 64); }
+template <typename T> __device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
+
+// read-only table entry at a wavefront-uniform address (scalar load)
+template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
+    static_assert(sizeof(S) % 4 == 0, "table entries are whole dwords");
+    typedef const __attribute__((address_space(4))) uint32_t cword;
+    cword *q = (cword *)p;
+    S out;
+    uint32_t *o = (uint32_t *)&out;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(S) / 4); i++) o[i] = q[i];
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Deep-water forms of the face fluxes: the 2s branch that mass_flux / mom_flux take
+// when all four stencil depths exceed hpos outside the first-order band
+// (src/GPU/Arista_kernel.cu:545,653,1105,1214), written with the per-cell square
+// roots and velocities computed once per cell, and the positivity limiter evaluated
+// without a division unless it acts.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T fast_positivity(T F, T lim0, T lim1, T dt) {
+    const T lim = (F > T(0)) ? lim0 : lim1;
+    const T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    if (dt * den > lim) {
+        const T dt0 = Num<T>::div(lim, den);
+        if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
+    }
+    return F;
+}
+
+// x face between cells a (left) and b: src/GPU/Arista_kernel.cu:79-95,118-130
+template <typename T>
+__device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T len, T g, T A, T dt) {
+    const T F = qa * len * g;
+    return fast_positivity(F, T(0.25) * ha * T(1) * A, T(0.25) * hb * T(1) * A, dt);
+}
+
+// y face between rows a (g-1) and b (g): src/GPU/Arista_kernel.cu:97-114,674-686
+template <typename T>
+__device__ __forceinline__ T fast_mass_y(T qa, T ha, T hb, const FaceGeoY<T> &f, T dt) {
+    const T F = f.len * qa * f.cos0;
+    return fast_positivity(F, T(0.25) * ha * f.cos0 * f.A0, T(0.25) * hb * f.cos1 * f.A1, dt);
+}
+
+// one cell of a deep momentum face: total depth, eta, depth, normal and tangential
+// discharge, sqrt(h), normal velocity
+template <typename T> struct DC {
+    T h, e, D, qn, qt, s, un;
+};
+
+// x face, 2s momentum flux: src/GPU/Arista_kernel.cu:221-280
+template <typename T>
+__device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T len, T g, T valf, T dt, T cd2s) {
+    MF<T> o;
+    const T hij = T(0.5) * (a.h + b.h);
+    const T uij = Num<T>::div(a.s * a.un + b.s * b.un, a.s + b.s + Num<T>::eps);
+    const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+    const T deta = deta_of(a.e, b.e, a.D, b.D);
+    const T val = valf * dt;
+    const bool up = uij < T(0);
+    const T uu = up ? b.un : a.un;
+    const T qnn = up ? b.qn : a.qn;
+    const T qtt = up ? b.qt : a.qt;
+    const T il = len * g;
+    const T fn = uu * qnn * il;
+    const T DES = cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
+    o.mn = fn + hij * deta * il - DES;
+    o.pn = -fn + DES;
+    const T ft = uu * qtt * il;
+    o.mt = ft;
+    o.pt = -ft;
+    o.w = il * lmax;
+    return o;
+}
+
+// y face, 2s momentum flux: src/GPU/Arista_kernel.cu:780-843
+template <typename T>
+__device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, const FaceGeoY<T> &f, T dt, T cd2s) {
+    MF<T> o;
+    const T g0 = f.cos0, g1 = f.cos1, ic0 = f.ic0, ic1 = f.ic1, len = f.len;
+    const T hij = T(0.5) * (a.h + b.h);
+    const T uij = Num<T>::div(a.s * a.un + b.s * b.un, a.s + b.s + Num<T>::eps);
+    const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+    const T deta = T(0.5) * (g0 + g1) * deta_of(a.e, b.e, a.D, b.D) - uij * uij * (g1 - g0);
+    const T val = f.valy * dt;
+    const bool up = uij < T(0);
+    const T uu = up ? b.un : a.un;
+    const T qnn = up ? b.qn : a.qn;
+    const T qtt = up ? b.qt : a.qt;
+    const T fn = uu * qnn * len;
+    T mn, pn;
+    if (up) { pn = -fn; mn = fn * (g1 * ic0); }
+    else { mn = fn; pn = -fn * g0 * ic1; }
+    mn += hij * deta * len * ic0;
+    const T DES = cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
+    o.mn = mn - DES * ic0;
+    o.pn = pn + DES * ic1;
+    const T ft = uu * qtt * len;
+    o.mt = ft;
+    o.pt = -ft;
+    o.w = len * lmax;
+    return o;
+}
+
+template <typename T> struct RowCell {
+    T e, D, qx, qy;
+};
+
+template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T qt) {
+    SC<T> c;
+    c.e = e;
+    c.D = D;
+    c.h = e + D;
+    c.qn = qn;
+    c.qt = qt;
+    return c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot) {
+    __shared__ BlockCtl<T> bc;
+    __shared__ float s_red[SW_WAVES];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // XCD-aware order of the workgroups (see k_step): chunks of whole strip rows
+    const int bid = blockIdx.x;
+    const int loc = bid >> 3, ch = loc / a.chunk;
+    const int tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
+
+    if (tid == 0) begin_step(a, slot, bid == 0, bc);
+    __syncthreads();
+    if (!bc.run || tile >= a.n_tiles) return;
+
+    const int cur = bc.cur;
+    const T *__restrict__ ge = a.e[cur];
+    const T *__restrict__ gqx = a.qx[cur];
+    const T *__restrict__ gqy = a.qy[cur];
+    const T *__restrict__ gD = a.D;
+    T *__restrict__ oe = a.e[cur ^ 1];
+    T *__restrict__ oqx = a.qx[cur ^ 1];
+    T *__restrict__ oqy = a.qy[cur ^ 1];
+    const T dt = (T)bc.dt;
+    const T tnow = (T)bc.t;
+    const int nx = a.nx, ny = a.ny, P = a.pitch;
+    const bool per = a.periodic != 0;
+
+    if (bid == 0) gather_gauges(a, bc, ge, gqx, gqy, tid, 64 * SW_WAVES);
+
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int x0 = (txi * SW_WAVES + wv) * SW_OUT;  // first output column of this wavefront
+    const int y0 = tyi * a.srows;
+    const int y1 = min(y0 + a.srows, ny);
+    float dtmin = 3.0e38f;
+
+    if (x0 < nx) {
+        const int gx = x0 - SW_HALO + lane;
+        // column loaded by this lane: wrapped (periodic) or clamped onto the grid
+        const int cx = per ? ((gx % nx) + nx) % nx : (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx));
+        // lanes outside the grid take the e* of the clamped column (clamped stencils)
+        const bool xedge = !per && (x0 - SW_HALO < 0 || x0 + SW_OUT + SW_HALO > nx);
+        const int src = cx - (x0 - SW_HALO);
+        // this lane's x face is the left face of its cell: g = gx
+        const int xb = per ? 0 : (gx == 0 ? 1 : (gx == nx ? 2 : 0));
+        const int gw = per ? cx : gx;
+        const bool xac = (gw < 2) || (gw > nx - 2);
+        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT && gx < nx;
+        const bool any_sponge = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
+        const int isp = per ? cx : gx;
+
+        Prm<T> px, py;
+        px.eps_h = py.eps_h = a.eps_h;
+        px.hpos = py.hpos = a.hpos;
+        px.cvis = py.cvis = a.cvis;
+        px.cd2s = py.cd2s = T(0.5) * (T(1) - a.cvis);
+        px.bl = a.bizq; px.bh = a.bder;
+        py.bl = a.bsup; py.bh = a.binf;
+
+        auto clamp_row = [&](int r) { return r < 0 ? 0 : (r >= ny ? ny - 1 : r); };
+        auto clamp_face = [&](int g) { return g < 0 ? 0 : (g > ny ? ny : g); };
+        // loads go through a wavefront-uniform row base plus the lane's column, and are
+        // never predicated (rows are clamped onto the grid), so the compiler can keep
+        // the prefetches in flight across iterations
+        const unsigned ucx = (unsigned)cx;
+        auto load_row = [&](int r) {
+            const int64_t o = (int64_t)clamp_row(r) * P;
+            RowCell<T> c;
+            c.e = (ge + o)[ucx];
+            c.D = (gD + o)[ucx];
+            c.qx = (gqx + o)[ucx];
+            c.qy = (gqy + o)[ucx];
+            return c;
+        };
+        // metrics inputs: without arrival tracking the arrival loads read emax instead
+        const T *__restrict__ gmax = a.emax;
+        const T *__restrict__ gtar = a.track_arrival ? a.tarr : a.emax;
+        const T *__restrict__ gini = a.track_arrival ? a.eini : a.emax;
+        const unsigned gxm = (unsigned)(gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx));
+        auto load_metrics = [&](int jn, T &m, T &t, T &i) {
+            const int64_t o = (int64_t)(jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P;
+            m = (gmax + o)[gxm];
+            t = (gtar + o)[gxm];
+            i = (gini + o)[gxm];
+        };
+
+        // the segment touches the first-order band of the x borders: no x face of it
+        // takes the deep form
+        const bool seg_ac = __any(xac) != 0;
+        const T cd2s = T(0.5) * (T(1) - a.cvis);
+        const T hpos = a.hpos;
+
+        // pipeline registers: input rows r-5..r, e* rows r-5..r-2 (with sqrt(h*) and
+        // qy/h* of rows r-4..r-2 for the deep y faces), mass accumulators rows
+        // r-2..r-1, momentum accumulators rows r-4..r-3, and per-row wavefront votes
+        // "every lane deeper than hpos" for the inputs (rows r-3..r) and h* (r-5..r-2)
+        RowCell<T> i5, i4, i3, i2, i1, i0;
+        i5 = i4 = i3 = i2 = i1 = i0 = RowCell<T>{T(0), T(0), T(0), T(0)};
+        T s5 = T(0), s4 = T(0), s3 = T(0);
+        T sq4 = T(0), sq3 = T(0), v4 = T(0), v3 = T(0);
+        T m2 = T(0), m1 = T(0);
+        T X4 = T(0), Y4 = T(0), W4 = T(0), X3 = T(0), Y3 = T(0), W3 = T(0);
+        bool din3 = false, din2 = false, din1 = false;
+        bool ds5 = false, ds4 = false, ds3 = false;
+        // One pipeline iteration for row r.  buf holds row r on entry and receives row
+        // r+1; (pm, pt, pi) hold the metrics inputs of row r-4 on entry and receive those
+        // of row r-3: every load has one iteration of work behind it.  (Keeping loads two
+        // iterations ahead needs a two-way unrolled loop, whose register use costs more
+        // occupancy than the extra latency cover gains.)
+        auto iteration = [&](const int r, RowCell<T> &buf, T &pm, T &pt, T &pi) {
+            i0 = buf;
+            buf = load_row(r + 1);
+            const int ju = r - 4;  // row updated in this iteration
+            const T cmax = pm, ctar = pt, cini = pi;
+            load_metrics(ju + 1, pm, pt, pi);
+            const T h0 = i0.e + i0.D;
+            const bool din0 = __all(h0 > hpos) != 0;
+
+            // ---- mass x-faces of row r ----
+            T m0;
+            {
+                const RowGeo<T> rg = uniform_load(a.row + clamp_row(r));
+                if (din0 && !seg_ac) {
+                    const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
+                    m0 = F - from_right(F);
+                } else {
+                    const T eL = from_left(i0.e), DL = from_left(i0.D), qL = from_left(i0.qx);
+                    const T eLL = from_left2(i0.e), DLL = from_left2(i0.D);
+                    const T eR = from_right(i0.e), DR = from_right(i0.D);
+                    const SC<T> M = sc_of(eLL, DLL, T(0), T(0)), A = sc_of(eL, DL, qL, T(0));
+                    const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0)), Q = sc_of(eR, DR, T(0), T(0));
+                    const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.lenx, rg.gx, rg.gx, rg.A, rg.A, rg.valx, dt, px);
+                    m0 = ((xb == 1) ? -F : F) - from_right(F);
+                }
+            }
+            // ---- mass y-face g = r-1 (rows r-3..r) ----
+            {
+                const int g = r - 1;
+                const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
+                const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
+                const bool yac = (g < 2) || (g > ny - 2);
+                T F;
+                if (din3 && din2 && din1 && din0 && !yac) {
+                    F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
+                } else {
+                    const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0));
+                    const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0)), Q = sc_of(i0.e, i0.D, T(0), T(0));
+                    F = mass_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.A0, fg.A1, fg.valy, dt, py);
+                }
+                m2 -= F * fg.ic0;
+                m1 += ((yb == 1) ? -F : F) * fg.ic1;
+            }
+            // ---- h* (as e*) of row r-2, with the sponge ----
+            T s2;
+            const RowGeo<T> rg2 = uniform_load(a.row + clamp_row(r - 2));
+            {
+                T es = i2.e + dt * rg2.invA * m2;
+                if (!(es + i2.D > T(0))) es = -i2.D;
+                if (any_sponge) es = sponge_eta(a, es, i2.D, isp, r - 2, rg2.dxs, rg2.dys);
+                if (xedge) es = __shfl(es, src, 64);
+                s2 = es;
+            }
+            const T hs2 = s2 + i2.D;
+            const bool ds2 = __all(hs2 > hpos) != 0;
+            T sq2 = T(0), u2 = T(0), v2 = T(0);
+            if (ds2) {
+                sq2 = Num<T>::sqrt(hs2);
+                const T ih = Num<T>::rcp(hs2);
+                u2 = i2.qx * ih;
+                v2 = i2.qy * ih;
+            }
+            // ---- momentum x-faces of row r-2 ----
+            T X2, Y2, W2;
+            if (ds2 && !seg_ac) {
+                DC<T> A, B;
+                A.h = from_left(hs2); A.e = from_left(s2); A.D = from_left(i2.D);
+                A.qn = from_left(i2.qx); A.qt = from_left(i2.qy); A.s = from_left(sq2); A.un = from_left(u2);
+                B.h = hs2; B.e = s2; B.D = i2.D; B.qn = i2.qx; B.qt = i2.qy; B.s = sq2; B.un = u2;
+                const MF<T> f = fast_mom_x(A, B, rg2.lenx, rg2.gx, rg2.valx, dt, cd2s);
+                X2 = -f.pn - from_right(f.mn);
+                Y2 = -f.pt - from_right(f.mt);
+                W2 = f.w + from_right(f.w);
+            } else {
+                const T eL = from_left(s2), DL = from_left(i2.D), qxL = from_left(i2.qx), qyL = from_left(i2.qy);
+                const T eLL = from_left2(s2), DLL = from_left2(i2.D), qyLL = from_left2(i2.qy);
+                const T eR = from_right(s2), DR = from_right(i2.D), qyR = from_right(i2.qy);
+                const SC<T> M = sc_of(eLL, DLL, T(0), qyLL), A = sc_of(eL, DL, qxL, qyL);
+                const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy), Q = sc_of(eR, DR, T(0), qyR);
+                const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.lenx, rg2.gx, rg2.gx, rg2.valx, dt, px);
+                X2 = -((xb == 1) ? f.mn : f.pn) - from_right(f.mn);
+                Y2 = -((xb == 1) ? f.mt : f.pt) - from_right(f.mt);
+                W2 = f.w + from_right(f.w);
+            }
+            // ---- momentum y-face g = r-3 (rows r-5..r-2) ----
+            {
+                const int g = r - 3;
+                const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
+                const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
+                const bool yac = (g < 2) || (g > ny - 2);
+                MF<T> f;
+                if (ds5 && ds4 && ds3 && ds2 && !yac) {
+                    DC<T> A, B;
+                    A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
+                    B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
+                    f = fast_mom_y(A, B, fg, dt, cd2s);
+                } else {
+                    SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx), A = sc_of(s4, i4.D, i4.qy, i4.qx);
+                    SC<T> B = sc_of(s3, i3.D, i3.qy, i3.qx), Q = sc_of(s2, i2.D, i2.qy, i2.qx);
+                    // rows outside the grid take the e* of the edge row (clamped stencil)
+                    if (g - 2 < 0) M = (g - 1 < 0) ? B : A;
+                    if (g - 1 < 0) A = B;
+                    if (g >= ny) B = A;
+                    if (g + 1 >= ny) Q = (g >= ny) ? A : B;
+                    f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.valy, dt, py);
+                }
+                Y4 -= f.mn;
+                X4 -= f.mt;
+                W4 += f.w;
+                if (yb == 1) { Y3 -= f.mn; X3 -= f.mt; }
+                else { Y3 -= f.pn; X3 -= f.pt; }
+                W3 += f.w;
+            }
+            // ---- cell update of row r-4 (src/GPU/Volumen_kernel.cu:160-221) ----
+            if (ju >= y0 && ju < y1 && out_col) {
+                const RowGeo<T> rg = uniform_load(a.row + ju);
+                const int64_t go = (int64_t)ju * P + gx;  // gx = gxm on output lanes
+                const T D = i4.D, eo = i4.e;
+                // metrics from the state before the step (src/GPU/Metrics.cu:23-42,57-77)
+                if (eo > cmax && eo + D > a.eps_h) a.emax[go] = eo;
+                if (a.track_arrival && ctar < T(0) && tabs(eo - cini) > a.dif_at) a.tarr[go] = tnow;
+                const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
+                dtmin = fminf(dtmin, (float)dtc);
+                const T val = dt * rg.invA;
+                T qx = i4.qx + val * X4;
+                T qy = i4.qy + val * Y4;
+                if (any_sponge) sponge_q(a, gx, ju, qx, qy);
+                const T hs = s4 + D;
+                if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
+                oe[go] = s4;
+                oqx[go] = qx;
+                oqy[go] = qy;
+            }
+            // ---- advance the pipeline ----
+            i5 = i4; i4 = i3; i3 = i2; i2 = i1; i1 = i0;
+            s5 = s4; s4 = s3; s3 = s2;
+            sq4 = sq3; sq3 = sq2; v4 = v3; v3 = v2;
+            m2 = m1; m1 = m0;
+            X4 = X3; Y4 = Y3; W4 = W3;
+            X3 = X2; Y3 = Y2; W3 = W2;
+            din3 = din2; din2 = din1; din1 = din0;
+            ds5 = ds4; ds4 = ds3; ds3 = ds2;
+        };
+
+        // rows y0-4 .. y1+3
+        const int rbeg = y0 - 4, rend = y1 + 3;
+        RowCell<T> buf = load_row(rbeg);
+        T pm, pt, pi;
+        load_metrics(rbeg - 4, pm, pt, pi);
+        for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt, pi);
+    }
+    dtmin = wave_min(dtmin);
+    if (lane == 0) s_red[wv] = dtmin;
+    __syncthreads();
+    if (tid == 0) {
+        float m = s_red[0];
+#pragma unroll
+        for (int w = 1; w < SW_WAVES; w++) m = fminf(m, s_red[w]);
+        atomicMin(&a.dtmin[slot], __float_as_uint(m));
+    }
+}
+
+}  // namespace thy

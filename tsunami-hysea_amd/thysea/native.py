@@ -33,7 +33,10 @@ class Params(C.Structure):
                 ("vmax", C.c_double), ("eps_h", C.c_double), ("hpos", C.c_double), ("cvis", C.c_double),
                 ("dif_at", C.c_double), ("hmin", C.c_double), ("sea_level", C.c_double), ("L", C.c_double),
                 ("H", C.c_double), ("Q", C.c_double), ("T", C.c_double), ("precision", C.c_int),
-                ("track_arrival", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int)]
+                ("track_arrival", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("kernel", C.c_int),
+                ("strip_rows", C.c_int)]
+
+KERNELS = {"sweep": 0, "tile": 1}
 
 
 class Fault(C.Structure):
@@ -98,7 +101,7 @@ def probe():
     return lib().thy_probe()
 
 
-def params_of(prob, precision=0, track_arrival=True, tile=(0, 0)):
+def params_of(prob, precision=0, track_arrival=True, tile=(0, 0), kernel="sweep", strip_rows=0):
     p = Params()
     p.borde_sup, p.borde_inf, p.borde_izq, p.borde_der = prob.borde_sup, prob.borde_inf, prob.borde_izq, prob.borde_der
     p.sponge_sup, p.sponge_inf = prob.sponge_sup, prob.sponge_inf
@@ -110,6 +113,8 @@ def params_of(prob, precision=0, track_arrival=True, tile=(0, 0)):
     p.precision = int(precision)
     p.track_arrival = 1 if track_arrival else 0
     p.tile_x, p.tile_y = int(tile[0]), int(tile[1])
+    p.kernel = KERNELS[kernel]
+    p.strip_rows = int(strip_rows)
     return p
 
 
@@ -123,12 +128,13 @@ def fault_of(f):
 class Solver:
     """One grid on the GPU, driven through the C-ABI."""
 
-    def __init__(self, prob, precision=0, track_arrival=True, tile=(0, 0), upload=True):
+    def __init__(self, prob, precision=0, track_arrival=True, tile=(0, 0), upload=True, kernel="sweep",
+                 strip_rows=0):
         self.prob = prob
         self._keep = [np.ascontiguousarray(a, np.float64) for a in
                       (prob.lon, prob.lat, prob.area, prob.cosphi, prob.ancho, prob.alto)]
         g = Grid(prob.nx, prob.ny, *[a.ctypes.data_as(_dp) for a in self._keep])
-        self.params = params_of(prob, precision, track_arrival, tile)
+        self.params = params_of(prob, precision, track_arrival, tile, kernel, strip_rows)
         h = C.c_void_p()
         rc = lib().thy_create(C.byref(g), C.byref(self.params), C.byref(h))
         self.h = h
