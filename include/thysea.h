@@ -179,6 +179,37 @@ int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 /* Wall time (ms) of the last kernel launch group, for benchmarking. */
 int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);
 
+/* ---- Batched scenarios --------------------------------------------------------
+   No counterpart in the reference, which runs one scenario per process
+   (src/GPU/ShallowWater.cu:182-190).  A batch advances many contexts together: each
+   step launch is one persistent kernel that takes (scenario, strip block) work items
+   from a device queue, and every scenario keeps its own dt, time, faults, gauges and
+   metrics, so its results are those of thy_run on the same context.  Contexts are
+   created and uploaded by the caller (thy_create, thy_upload_state), admitted with
+   their run configuration, and stay owned by the caller; results are read with
+   thy_get_field / thy_get_gauges once the slot is THY_SLOT_FINISHED.  FP32 row-sweep
+   contexts only; no snapshots (save_interval < 0). */
+typedef struct thy_batch thy_batch;
+
+enum thy_slot_state { THY_SLOT_FREE = 0, THY_SLOT_RUNNING = 1, THY_SLOT_FINISHED = 2 };
+
+/* capacity scenario slots; hbm_budget bytes of scenario state (0: unlimited);
+   steps_per_sync launches per captured step sequence (rounded up to a multiple of 3). */
+int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_batch **out);
+void thy_batch_destroy(thy_batch *b);
+const char *thy_batch_last_error(thy_batch *b);
+/* Prepares the run (metrics, initial dt, faults due at t = 0) and admits the context;
+   THY_ERR_BUDGET when no slot is free or its footprint would exceed the budget. */
+int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot);
+/* One captured sequence of steps_per_sync launches for every running scenario, then
+   the host work between them (faults that came due, finished scenarios). */
+int thy_batch_advance(thy_batch *b, int *n_running);
+int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res);
+/* Frees a slot (the context is not destroyed; remove a running scenario's slot before
+   destroying its context). */
+int thy_batch_remove(thy_batch *b, int slot);
+int thy_batch_info(thy_batch *b, int *running, size_t *bytes_used, double *gpu_seconds);
+
 #ifdef __cplusplus
 }
 #endif

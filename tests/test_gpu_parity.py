@@ -147,3 +147,78 @@ def test_full_run_small_matches_oracle(kw):
     n = min(len(rg["g_time"]), len(ro["g_time"]))
     assert n > 10
     assert rel_linf(rg["g_eta"][:n], ro["g_eta"][:n], peak) < 2e-3
+
+
+def _scenario(nx, ny, seed, fault_frac, slip, t_sim):
+    pr = make_problem(nx=nx, ny=ny, seed=seed, t_sim=t_sim)
+    f = dict(pr.faults[0])
+    f["lon_c"] = float(pr.lon[int(nx * fault_frac)])
+    f["slip"] = slip
+    pr.faults = [f]
+    from thysea import scenario as S
+    from thysea import problem as P
+    glon, glat = S.gauges_lattice(pr.lon, pr.lat, 2)
+    pr.gauge_pos = np.array([P.nearest_cell(pr.lon, pr.lat, a, b) for a, b in zip(glon, glat)], np.int32)
+    pr.series_interval = 20.0 / pr.T
+    return pr
+
+
+def test_batch_matches_alone():
+    """Scenarios of different sizes and durations, advanced together in one batch,
+    give the results of each run alone (same kernel code, own dt per scenario): max
+    height, arrival times, gauges and step counts identical; one scenario admitted
+    while the others are in flight."""
+    nat = _native()
+    probs = [_scenario(96, 80, 3, 0.4, 2.0, 600.0), _scenario(150, 64, 5, 0.5, 4.0, 900.0),
+             _scenario(70, 120, 9, 0.3, 1.0, 400.0)]
+    alone = []
+    for pr in probs:
+        s = nat.Solver(pr)
+        r = s.run(steps_per_sync=6)
+        r["max_height"] = s.field("max_height")
+        r["arrival_times"] = s.field("arrival_times")
+        alone.append(r)
+        s.close()
+    b = nat.Batch(capacity=4, steps_per_sync=6)
+    solvers = [nat.Solver(pr) for pr in probs]
+    slots = [b.add(solvers[0]), b.add(solvers[1])]
+    b.advance()
+    slots.append(b.add(solvers[2]))  # admitted mid-flight
+    for _ in range(10000):
+        if b.advance() == 0:
+            break
+    assert sorted(b.finished()) == sorted(slots)
+    for slot, ra in zip(slots, alone):
+        rb = b.result(slot)
+        assert rb["steps"] == ra["steps"]
+        assert np.array_equal(rb["max_height"], ra["max_height"])
+        assert np.array_equal(rb["arrival_times"], ra["arrival_times"])
+        assert rb["n_samples"] == ra["n_samples"] and rb["n_samples"] > 5
+        assert np.array_equal(rb["g_eta"], ra["g_eta"])
+        b.remove(slot)
+    assert b.info()["running"] == 0 and b.info()["bytes"] == 0
+    b.close()
+    for s in solvers:
+        s.close()
+
+
+def test_batch_budget_admission():
+    """A scenario beyond the HBM budget is refused (None), and admitted once a
+    finished one has been removed."""
+    nat = _native()
+    pr = _scenario(96, 80, 3, 0.4, 2.0, 120.0)
+    need = nat.footprint(pr.nx, pr.ny)
+    b = nat.Batch(capacity=4, hbm_budget=int(need * 1.5), steps_per_sync=6)
+    s1, s2 = nat.Solver(pr), nat.Solver(pr)
+    a = b.add(s1)
+    assert a is not None
+    assert b.add(s2) is None
+    while b.advance() > 0:
+        pass
+    b.remove(a)
+    assert b.add(s2) is not None
+    while b.advance() > 0:
+        pass
+    b.close()
+    s1.close()
+    s2.close()

@@ -54,7 +54,10 @@ struct thy_ctx {
     int tx = 64, ty = 16;
     int chunk_rows = 1;          // tile (strip) rows per XCD chunk of the step kernel's order
     int kernel = 0;              // 0 row sweep, 1 2D tiles
-    int srows = 64;              // row sweep: rows per strip
+    int srows = 32;              // row sweep: rows per strip
+    int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
+    int order_n = 0;
+    std::vector<int> order_h, cls_h;  // host copies: the order and each block's cost class
     hipGraphExec_t gexec = nullptr;
     int gk = 0;
     double last_ms = 0.0;
@@ -116,6 +119,7 @@ StepArgs<T> make_args(thy_ctx *c) {
         a.tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES;
         a.tiles_y = (c->ny + c->srows - 1) / c->srows;
         a.srows = c->srows;
+        a.order = c->order;
     } else {
         a.tiles_x = (c->nx + c->tx - 1) / c->tx;
         a.tiles_y = (c->ny + c->ty - 1) / c->ty;
@@ -155,6 +159,7 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
     const int n_chunks = (a.tiles_y + c->chunk_rows - 1) / c->chunk_rows;
     dim3 grid(((n_chunks + 7) / 8) * 8 * a.chunk), block(256);
     if (c->kernel == 0) {
+        if (a.order) grid = dim3(a.n_tiles);
         hipLaunchKernelGGL((k_sweep<T>), grid, dim3(64 * SW_WAVES), 0, c->stream, a, slot);
         return hipGetLastError();
     }
@@ -248,6 +253,78 @@ int pick_tile(thy_ctx *c) {
     return THY_OK;
 }
 
+// Dispatch order of the row-sweep workgroups.  A strip row whose stencil touches
+// shallow water or land runs the general faces, about 2.5x the work of a deep row, so
+// coastal strips are the longest-running workgroups; dispatching them first lets the
+// deep ones fill in behind them instead of leaving a tail of coastal strips at the end
+// of the launch.  Workgroups of equal cost class keep the chunked XCD order of the
+// kernel.  The estimate uses the still-water depth: a row counts as coastal for a
+// 64-column segment when any cell of it is shallower than 3 hpos (or dry).
+template <typename T>
+int build_order(thy_ctx *ctx, const std::vector<T> &D) {
+    thy_ctx *c = ctx;
+    if (c->kernel != 0) return THY_OK;
+    const int nx = c->nx, ny = c->ny, P = c->pitch, S = c->srows;
+    const int nseg = (nx + SW_OUT - 1) / SW_OUT;
+    const int tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES, tiles_y = (ny + S - 1) / S;
+    const int n_tiles = tiles_x * tiles_y;
+    const double thr = 3.0 * c->prm.hpos;
+    // coastal[j * nseg + s]: row j has a shallow or dry cell within segment s's 64 columns
+    std::vector<unsigned char> coastal((size_t)ny * nseg, 0);
+    for (int j = 0; j < ny; j++) {
+        for (int sgi = 0; sgi < nseg; sgi++) {
+            const int xa = std::max(0, sgi * SW_OUT - SW_HALO), xb = std::min(nx, sgi * SW_OUT + SW_OUT + SW_HALO);
+            bool any = false;
+            for (int i = xa; i < xb && !any; i++) any = (double)D[(size_t)j * P + i] <= thr;
+            coastal[(size_t)j * nseg + sgi] = any ? 1 : 0;
+        }
+    }
+    std::vector<int> cls(n_tiles, 0);
+    for (int t = 0; t < n_tiles; t++) {
+        const int ty = t / tiles_x, tx = t - ty * tiles_x;
+        const int y0 = ty * S, y1 = std::min(y0 + S, ny);
+        const int rows = (y1 - y0) + 8;
+        double worst = 0.0;
+        for (int w = 0; w < SW_WAVES; w++) {
+            const int sgi = tx * SW_WAVES + w;
+            if (sgi >= nseg) break;
+            int nc = 0;
+            for (int j = std::max(0, y0 - 4); j < std::min(ny, y1 + 4); j++) nc += coastal[(size_t)j * nseg + sgi];
+            worst = std::max(worst, 1.5 * nc / rows);
+        }
+        cls[t] = (int)(worst * 4.0 + 0.5);
+    }
+    // chunked XCD order as the base sequence (the kernel's order without a table)
+    std::vector<int> base;
+    base.reserve(n_tiles);
+    const int chunk = tiles_x * c->chunk_rows;
+    const int n_chunks = (tiles_y + c->chunk_rows - 1) / c->chunk_rows;
+    const int grid = ((n_chunks + 7) / 8) * 8 * chunk;
+    for (int bid = 0; bid < grid; bid++) {
+        const int loc = bid >> 3, ch = loc / chunk;
+        const int t = ((ch << 3) + (bid & 7)) * chunk + (loc - ch * chunk);
+        if (t < n_tiles) base.push_back(t);
+    }
+    std::stable_sort(base.begin(), base.end(), [&](int a, int b) { return cls[a] > cls[b]; });
+    c->order_h = base;
+    c->cls_h = cls;
+    if (c->gexec) {  // a captured step sequence holds the old table pointer and grid
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        c->gk = 0;
+    }
+    if (c->order && c->order_n != n_tiles) {
+        HIPCHK(hipFree(c->order));
+        c->order = nullptr;
+    }
+    if (!c->order) {
+        HIPCHK(hipMalloc(&c->order, (size_t)n_tiles * sizeof(int)));
+        c->order_n = n_tiles;
+    }
+    HIPCHK(hipMemcpy(c->order, base.data(), (size_t)n_tiles * sizeof(int), hipMemcpyHostToDevice));
+    return THY_OK;
+}
+
 template <typename T>
 int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
@@ -273,7 +350,7 @@ int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     HIPCHK(hipMemcpy(ctx->qx[1], qx.data(), b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->qy[1], qy.data(), b, hipMemcpyHostToDevice));
     ctx->cur = 0;
-    return THY_OK;
+    return build_order<T>(ctx, D);
 }
 
 template <typename T>
@@ -646,7 +723,7 @@ void thy_destroy(thy_ctx *ctx) {
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
-                    ctx->steplog};
+                    ctx->steplog, ctx->order};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -780,6 +857,8 @@ static int drain_log(thy_ctx *ctx, int64_t upto) {
     return THY_OK;
 }
 
+static int collect_gauges(thy_ctx *ctx);
+
 static int run_loop(thy_ctx *ctx, thy_run_result *res) {
     int r;
     double total_ms = 0.0;
@@ -821,7 +900,11 @@ static int run_loop(thy_ctx *ctx, thy_run_result *res) {
     res->n_snapshots = ctx->nsnap;
     res->seconds = total_ms * 1e-3;
     ctx->last_ms = total_ms;
-    // collect gauge samples
+    return collect_gauges(ctx);
+}
+
+// gauge samples of the finished run into host memory (read by thy_get_gauges)
+static int collect_gauges(thy_ctx *ctx) {
     const int ns = ctx->nsamples, ng = ctx->ng;
     ctx->g_time.assign(ns, 0.f);
     ctx->g_eta.assign((size_t)ns * ng, 0.f);
@@ -840,8 +923,9 @@ static int run_loop(thy_ctx *ctx, thy_run_result *res) {
     return THY_OK;
 }
 
-int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
-    if (!ctx || !cfg || !res) return THY_ERR_INVALID;
+// Everything thy_run does before its first step launch: schedule, gauge buffers,
+// metrics, initial dt and truncation, control block in launch slot 0.
+static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     if (cfg->n_faults < 0 || (cfg->n_faults > 0 && !cfg->faults) || (cfg->n_gauges > 0 && !cfg->gauge_pos))
         return fail(ctx, THY_ERR_INVALID, "bad run configuration");
     int r;
@@ -917,8 +1001,16 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
     c.skip = skip;
     c.cur = ctx->cur;
     if ((r = reset_dt_slots(ctx)) != THY_OK) return r;
+    ctx->slot = 0;  // the next launch reads slot 0 (the dt slots were all reset)
     if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
     ctx->run_active = true;
+    return THY_OK;
+}
+
+int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
+    if (!ctx || !cfg || !res) return THY_ERR_INVALID;
+    int r;
+    if ((r = prepare_run(ctx, cfg, res)) != THY_OK) return r;
     double dti = res->dt_initial;
     r = run_loop(ctx, res);
     res->dt_initial = dti;
@@ -935,6 +1027,288 @@ int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res) {
     c.max_steps = c.steps + n_steps;
     if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
     return run_loop(ctx, res);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched scenarios: many contexts advanced by one persistent launch per step.
+// ===========================================================================
+struct thy_batch {
+    int capacity = 0;
+    size_t budget = 0, used = 0;
+    int K = 24;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<thy_ctx *> ctx;     // per slot; nullptr when free
+    std::vector<size_t> bytes;      // admitted footprint per slot
+    std::vector<int> state;         // THY_SLOT_*
+    std::vector<thy_run_result> res;
+    StepArgs<float> *d_args = nullptr;
+    int2 *d_items = nullptr;
+    int items_cap = 0;
+    int *d_nitems = nullptr;
+    unsigned int *d_counter = nullptr;
+    Ctrl *d_ctrl = nullptr;
+    std::vector<Ctrl> h_ctrl;
+    hipGraphExec_t gexec = nullptr;
+    int grid = 0;
+    int n_items = 0;
+    bool dirty = true;
+    double gpu_ms = 0.0;
+    std::string err;
+};
+
+namespace {
+
+int bfail(thy_batch *b, int code, const std::string &msg) {
+    b->err = msg;
+    return code;
+}
+
+#define BHIPCHK(x)                                                                                        \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess)                                                                            \
+            return bfail(b, e_ == hipErrorOutOfMemory ? THY_ERR_DEVICE_MEMORY : THY_ERR_HIP,             \
+                         std::string(#x) + ": " + hipGetErrorString(e_));                                \
+    } while (0)
+
+// Work items of the running scenarios: costliest cost class first, then by scenario
+// and the scenario's own dispatch order.
+int batch_items(thy_batch *b) {
+    struct It { int cls, sc, pos, tile; };
+    std::vector<It> v;
+    for (int i = 0; i < b->capacity; i++) {
+        if (b->state[i] != THY_SLOT_RUNNING) continue;
+        thy_ctx *c = b->ctx[i];
+        for (size_t p = 0; p < c->order_h.size(); p++) {
+            const int t = c->order_h[p];
+            v.push_back({c->cls_h[t], i, (int)p, t});
+        }
+    }
+    std::stable_sort(v.begin(), v.end(), [](const It &x, const It &y) { return x.cls > y.cls; });
+    std::vector<int2> items(v.size());
+    for (size_t k = 0; k < v.size(); k++) items[k] = make_int2(v[k].sc, v[k].tile);
+    if ((int)items.size() > b->items_cap) {
+        if (b->d_items) BHIPCHK(hipFree(b->d_items));
+        b->d_items = nullptr;
+        b->items_cap = std::max<int>((int)items.size(), 2 * b->items_cap);
+        BHIPCHK(hipMalloc(&b->d_items, (size_t)b->items_cap * sizeof(int2)));
+        if (b->gexec) {  // the captured launches hold the old item array
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+    }
+    b->n_items = (int)items.size();
+    if (!items.empty())
+        BHIPCHK(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BHIPCHK(hipMemcpy(b->d_nitems, &b->n_items, sizeof(int), hipMemcpyHostToDevice));
+    b->dirty = false;
+    return THY_OK;
+}
+
+// K launches (slots 0, 1, 2, ...) of the persistent step kernel, captured once
+int batch_graph(thy_batch *b) {
+    if (b->gexec) return THY_OK;
+    QueueArgs q;
+    q.args = b->d_args;
+    q.items = b->d_items;
+    q.n_items = b->d_nitems;
+    q.counter = b->d_counter;
+    hipGraph_t g;
+    BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < b->K; k++)
+        hipLaunchKernelGGL(k_sweep_queue, dim3(b->grid), dim3(64 * SW_WAVES), 0, b->stream, q, k % 3);
+    BHIPCHK(hipStreamEndCapture(b->stream, &g));
+    BHIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return THY_OK;
+}
+
+// apply every fault of a scenario whose time has come; Ctrl c is its control block
+int batch_faults(thy_batch *b, thy_ctx *c, Ctrl &cb, bool &changed) {
+    changed = false;
+    while (cb.k < cb.nf && cb.t * cb.T >= c->ftimes[cb.k]) {
+        const int r = thy_apply_okada(c, &c->faults[cb.k], c->crop_flag, c->crop_value, nullptr);
+        if (r != THY_OK) return bfail(b, r, std::string("okada: ") + thy_last_error(c));
+        cb.k++;
+        changed = true;
+    }
+    if (changed) {
+        BHIPCHK(hipStreamSynchronize(c->stream));
+        c->slot = 0;
+        const int r = write_ctrl(c, cb);
+        if (r != THY_OK) return bfail(b, r, thy_last_error(c));
+    }
+    return THY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_batch **out) {
+    if (!out || capacity < 1 || capacity > 65536) return THY_ERR_INVALID;
+    thy_batch *b = new (std::nothrow) thy_batch();
+    if (!b) return THY_ERR_HOST_MEMORY;
+    *out = b;
+    b->capacity = capacity;
+    b->budget = hbm_budget;
+    b->K = steps_per_sync > 0 ? ((steps_per_sync + 2) / 3) * 3 : 24;
+    b->ctx.assign(capacity, nullptr);
+    b->bytes.assign(capacity, 0);
+    b->state.assign(capacity, THY_SLOT_FREE);
+    b->res.assign(capacity, thy_run_result{});
+    b->h_ctrl.resize(capacity);
+    BHIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BHIPCHK(hipEventCreate(&b->ev0));
+    BHIPCHK(hipEventCreate(&b->ev1));
+    BHIPCHK(hipMalloc(&b->d_args, (size_t)capacity * sizeof(StepArgs<float>)));
+    BHIPCHK(hipMemset(b->d_args, 0, (size_t)capacity * sizeof(StepArgs<float>)));
+    BHIPCHK(hipMalloc(&b->d_nitems, sizeof(int)));
+    BHIPCHK(hipMemset(b->d_nitems, 0, sizeof(int)));
+    BHIPCHK(hipMalloc(&b->d_counter, 3 * sizeof(unsigned int)));
+    BHIPCHK(hipMemset(b->d_counter, 0, 3 * sizeof(unsigned int)));
+    BHIPCHK(hipMalloc(&b->d_ctrl, (size_t)capacity * sizeof(Ctrl)));
+    b->items_cap = 1024;
+    BHIPCHK(hipMalloc(&b->d_items, (size_t)b->items_cap * sizeof(int2)));
+    int dev = 0, ncu = 0, per_cu = 0;
+    BHIPCHK(hipGetDevice(&dev));
+    BHIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_queue, 64 * SW_WAVES, 0));
+    b->grid = std::max(1, ncu * std::max(1, per_cu));
+    return THY_OK;
+}
+
+void thy_batch_destroy(thy_batch *b) {
+    if (!b) return;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    void *ptrs[] = {b->d_args, b->d_items, b->d_nitems, b->d_counter, b->d_ctrl};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *thy_batch_last_error(thy_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot_out) {
+    if (!b || !ctx || !cfg || !slot_out) return THY_ERR_INVALID;
+    if (ctx->prec != 0 || ctx->kernel != 0)
+        return bfail(b, THY_ERR_INVALID, "batched scenarios run the FP32 row-sweep kernel only");
+    if (cfg->save_interval >= 0.0) return bfail(b, THY_ERR_INVALID, "batched scenarios take no snapshots");
+    if (ctx->order_h.empty()) return bfail(b, THY_ERR_INVALID, "scenario state not uploaded");
+    for (int i = 0; i < b->capacity; i++)
+        if (b->ctx[i] == ctx && b->state[i] != THY_SLOT_FREE) return bfail(b, THY_ERR_INVALID, "context already admitted");
+    int slot = -1;
+    for (int i = 0; i < b->capacity && slot < 0; i++)
+        if (b->state[i] == THY_SLOT_FREE) slot = i;
+    if (slot < 0) return bfail(b, THY_ERR_BUDGET, "no free scenario slot");
+    const size_t need = thy_footprint(ctx->nx, ctx->ny, &ctx->prm);
+    if (b->budget > 0 && b->used + need > b->budget) return bfail(b, THY_ERR_BUDGET, "HBM budget exceeded");
+    int r;
+    thy_run_result res;
+    if ((r = prepare_run(ctx, cfg, &res)) != THY_OK) return bfail(b, r, thy_last_error(ctx));
+    Ctrl cb;
+    if ((r = read_ctrl(ctx, &cb)) != THY_OK) return bfail(b, r, thy_last_error(ctx));
+    bool changed;
+    if ((r = batch_faults(b, ctx, cb, changed)) != THY_OK) return r;
+    StepArgs<float> a = make_args<float>(ctx);
+    BHIPCHK(hipMemcpy(b->d_args + slot, &a, sizeof(a), hipMemcpyHostToDevice));
+    b->ctx[slot] = ctx;
+    b->bytes[slot] = need;
+    b->used += need;
+    b->state[slot] = THY_SLOT_RUNNING;
+    b->res[slot] = res;
+    b->dirty = true;
+    *slot_out = slot;
+    return THY_OK;
+}
+
+int thy_batch_advance(thy_batch *b, int *n_running) {
+    if (!b) return THY_ERR_INVALID;
+    int r;
+    if (b->dirty && (r = batch_items(b)) != THY_OK) return r;
+    int hi = 0, running = 0;
+    for (int i = 0; i < b->capacity; i++)
+        if (b->state[i] == THY_SLOT_RUNNING) {
+            hi = i + 1;
+            running++;
+        }
+    if (n_running) *n_running = running;
+    if (running == 0) return THY_OK;
+    if ((r = batch_graph(b)) != THY_OK) return r;
+    BHIPCHK(hipEventRecord(b->ev0, b->stream));
+    BHIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    BHIPCHK(hipEventRecord(b->ev1, b->stream));
+    // K is a multiple of 3: the next launch reads slot 0 again
+    hipLaunchKernelGGL(k_gather_ctrl, dim3((hi + 63) / 64), dim3(64), 0, b->stream, b->d_args, hi, 0, b->d_ctrl);
+    BHIPCHK(hipGetLastError());
+    BHIPCHK(hipMemcpyAsync(b->h_ctrl.data(), b->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->gpu_ms += ms;
+    for (int i = 0; i < hi; i++) {
+        if (b->state[i] != THY_SLOT_RUNNING) continue;
+        thy_ctx *c = b->ctx[i];
+        Ctrl cb = b->h_ctrl[i];
+        c->cur = cb.cur;
+        if (!cb.dt_valid) continue;  // stepped through the whole replay
+        // stopped: finished, or a fault is due (the only other stop a batched run has)
+        if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {
+            c->nsamples = std::min(cb.nsamples, c->gcap);
+            if ((r = collect_gauges(c)) != THY_OK) return bfail(b, r, thy_last_error(c));
+            thy_run_result &res = b->res[i];
+            res.steps = cb.steps;
+            res.t_final = cb.t;
+            res.n_samples = c->nsamples;
+            b->state[i] = THY_SLOT_FINISHED;
+            b->dirty = true;
+            running--;
+            // no launch or gather reads a finished scenario again
+            BHIPCHK(hipMemset(b->d_args + i, 0, sizeof(StepArgs<float>)));
+            continue;
+        }
+        bool changed;
+        if ((r = batch_faults(b, c, cb, changed)) != THY_OK) return r;
+        if (!changed) return bfail(b, THY_ERR_INVALID, "batched scenario stalled");
+    }
+    if (n_running) *n_running = running;
+    return THY_OK;
+}
+
+int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res) {
+    if (!b || slot < 0 || slot >= b->capacity) return THY_ERR_INVALID;
+    if (state) *state = b->state[slot];
+    if (res) *res = b->res[slot];
+    return THY_OK;
+}
+
+int thy_batch_remove(thy_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= b->capacity || b->state[slot] == THY_SLOT_FREE) return THY_ERR_INVALID;
+    BHIPCHK(hipStreamSynchronize(b->stream));
+    BHIPCHK(hipMemset(b->d_args + slot, 0, sizeof(StepArgs<float>)));
+    b->ctx[slot]->run_active = false;
+    b->used -= b->bytes[slot];
+    b->bytes[slot] = 0;
+    b->ctx[slot] = nullptr;
+    b->state[slot] = THY_SLOT_FREE;
+    b->dirty = true;
+    return THY_OK;
+}
+
+int thy_batch_info(thy_batch *b, int *running, size_t *bytes_used, double *gpu_seconds) {
+    if (!b) return THY_ERR_INVALID;
+    int n = 0;
+    for (int i = 0; i < b->capacity; i++) n += b->state[i] == THY_SLOT_RUNNING;
+    if (running) *running = n;
+    if (bytes_used) *bytes_used = b->used;
+    if (gpu_seconds) *gpu_seconds = b->gpu_ms * 1e-3;
+    return THY_OK;
 }
 
 }  // extern "C"

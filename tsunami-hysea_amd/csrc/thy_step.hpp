@@ -56,6 +56,7 @@ template <typename T> struct StepArgs {
     int nx, ny, pitch, periodic;
     int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
     int srows;                             // row-sweep kernel: rows per strip
+    const int *order;                      // row-sweep kernel: workgroup -> block (or null)
     int sp_sup, sp_inf, sp_izq, sp_der;
     int track_arrival, n_gauges;
     T bsup, binf, bizq, bder;

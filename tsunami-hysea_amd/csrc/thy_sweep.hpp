@@ -39,12 +39,8 @@ template <typename T> __device__ __forceinline__ T from_right(T v) { return __sh
 // read-only table entry at a wavefront-uniform address (scalar load)
 template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
     static_assert(sizeof(S) % 4 == 0, "table entries are whole dwords");
-    typedef const __attribute__((address_space(4))) uint32_t cword;
-    cword *q = (cword *)p;
     S out;
-    uint32_t *o = (uint32_t *)&out;
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(S) / 4); i++) o[i] = q[i];
+    __builtin_memcpy(&out, (const __attribute__((address_space(4))) S *)p, sizeof(S));
     return out;
 }
 
@@ -154,36 +150,31 @@ template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T q
     return c;
 }
 
+// One workgroup's share of a step: strip block `tile` of the grid described by `a`
+// (SW_WAVES wavefronts side by side, each a 64-column strip of a.srows rows), with
+// bc filled by begin_step for this launch.  Returns this lane's minimum of the per-cell
+// dt of the next step (3e38 when it updated no cell).
 template <typename T>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot) {
-    __shared__ BlockCtl<T> bc;
-    __shared__ float s_red[SW_WAVES];
+__device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCtl<T> &bc, const int tile,
+                                             const bool gauges) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // XCD-aware order of the workgroups (see k_step): chunks of whole strip rows
-    const int bid = blockIdx.x;
-    const int loc = bid >> 3, ch = loc / a.chunk;
-    const int tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
-
-    if (tid == 0) begin_step(a, slot, bid == 0, bc);
-    __syncthreads();
-    if (!bc.run || tile >= a.n_tiles) return;
-
-    const int cur = bc.cur;
-    const T *__restrict__ ge = a.e[cur];
-    const T *__restrict__ gqx = a.qx[cur];
-    const T *__restrict__ gqy = a.qy[cur];
+    // (selects rather than indexing, so that a copy of the arguments stays in registers)
+    const bool c1 = bc.cur != 0;
+    const T *__restrict__ ge = c1 ? a.e[1] : a.e[0];
+    const T *__restrict__ gqx = c1 ? a.qx[1] : a.qx[0];
+    const T *__restrict__ gqy = c1 ? a.qy[1] : a.qy[0];
     const T *__restrict__ gD = a.D;
-    T *__restrict__ oe = a.e[cur ^ 1];
-    T *__restrict__ oqx = a.qx[cur ^ 1];
-    T *__restrict__ oqy = a.qy[cur ^ 1];
+    T *__restrict__ oe = c1 ? a.e[0] : a.e[1];
+    T *__restrict__ oqx = c1 ? a.qx[0] : a.qx[1];
+    T *__restrict__ oqy = c1 ? a.qy[0] : a.qy[1];
     const T dt = (T)bc.dt;
     const T tnow = (T)bc.t;
     const int nx = a.nx, ny = a.ny, P = a.pitch;
     const bool per = a.periodic != 0;
 
-    if (bid == 0) gather_gauges(a, bc, ge, gqx, gqy, tid, 64 * SW_WAVES);
+    if (gauges) gather_gauges(a, bc, ge, gqx, gqy, tid, 64 * SW_WAVES);
 
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int x0 = (txi * SW_WAVES + wv) * SW_OUT;  // first output column of this wavefront
@@ -415,8 +406,15 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
         load_metrics(rbeg - 4, pm, pt, pi);
         for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt, pi);
     }
+    return dtmin;
+}
+
+// workgroup minimum of the lanes' dt, folded into the launch's dt slot of the grid
+template <typename T>
+__device__ __forceinline__ void block_dt_min(const StepArgs<T> &a, const int slot, float dtmin, float *s_red) {
+    const int tid = threadIdx.x;
     dtmin = wave_min(dtmin);
-    if (lane == 0) s_red[wv] = dtmin;
+    if ((tid & 63) == 0) s_red[tid >> 6] = dtmin;
     __syncthreads();
     if (tid == 0) {
         float m = s_red[0];
@@ -424,6 +422,84 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
         for (int w = 1; w < SW_WAVES; w++) m = fminf(m, s_red[w]);
         atomicMin(&a.dtmin[slot], __float_as_uint(m));
     }
+}
+
+// One step of one grid: one workgroup per strip block.
+template <typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot) {
+    __shared__ BlockCtl<T> bc;
+    __shared__ float s_red[SW_WAVES];
+    const int tid = threadIdx.x;
+    // workgroup order: the host's cost-ordered table when present, else the XCD-aware
+    // chunked order of k_step (chunks of whole strip rows)
+    const int bid = blockIdx.x;
+    int tile;
+    if (a.order) {
+        tile = a.order[bid];
+    } else {
+        const int loc = bid >> 3, ch = loc / a.chunk;
+        tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
+    }
+    if (tid == 0) begin_step(a, slot, bid == 0, bc);
+    __syncthreads();
+    if (!bc.run || tile >= a.n_tiles) return;
+    const float dtmin = sweep_block(a, bc, tile, bid == 0);
+    block_dt_min(a, slot, dtmin, s_red);
+}
+
+}  // namespace thy
+
+namespace thy {
+
+// ---------------------------------------------------------------------------
+// Many grids per launch.  Each scenario keeps its own state, control block and dt
+// slots; a launch advances every scenario of the item list by one of its own steps.
+// Work items are (scenario, strip block) pairs, costliest first; a fixed set of
+// resident workgroups takes them from a device counter, so the launch balances
+// itself over scenarios of different sizes and finishing times, and the same
+// captured launch sequence serves a changing item list (the count is read on the
+// device).
+// ---------------------------------------------------------------------------
+struct QueueArgs {
+    const StepArgs<float> *args;  // one per scenario slot
+    const int2 *items;            // (scenario slot, strip block)
+    const int *n_items;
+    unsigned int *counter;        // 3 slots, launch n uses slot n mod 3
+};
+
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_queue(const QueueArgs q, const int slot) {
+    __shared__ BlockCtl<float> bc;
+    __shared__ float s_red[SW_WAVES];
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    const int n_items = *q.n_items;
+    for (;;) {
+        if (tid == 0) {
+            const int it = (int)atomicAdd(&q.counter[slot], 1u);
+            s_item = it;
+            if (it == 0) q.counter[(slot + 1) % 3] = 0u;  // the next launch's counter
+        }
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= n_items) break;
+        const int2 w = q.items[it];
+        const int sc = __builtin_amdgcn_readfirstlane(w.x), tile = __builtin_amdgcn_readfirstlane(w.y);
+        const StepArgs<float> a = uniform_load(q.args + sc);
+        if (tid == 0) begin_step(a, slot, tile == 0, bc);
+        __syncthreads();
+        if (bc.run) {
+            const float dtmin = sweep_block(a, bc, tile, tile == 0);
+            block_dt_min(a, slot, dtmin, s_red);
+        }
+        __syncthreads();  // bc, s_red and s_item are reused by the next item
+    }
+}
+
+// control blocks of the scenarios at launch slot `slot`, gathered for one copy (a free
+// scenario slot has a zeroed argument block and is skipped)
+__global__ void k_gather_ctrl(const StepArgs<float> *args, const int n, const int slot, Ctrl *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && args[i].ctrl) out[i] = args[i].ctrl[slot];
 }
 
 }  // namespace thy

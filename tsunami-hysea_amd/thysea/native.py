@@ -19,6 +19,8 @@ _ip = C.POINTER(C.c_int)
 FIELDS = {"eta": 0, "ux": 1, "uy": 2, "deformed_bathy": 3, "max_height": 4, "arrival_times": 5}
 STATUS = {0: "ok", 1: "device memory", 2: "host memory", 3: "invalid argument", 4: "no device",
           5: "HIP error", 6: "memory budget", 7: "non-finite state"}
+STATUS_CODES = {"OK": 0, "DEVICE_MEMORY": 1, "HOST_MEMORY": 2, "INVALID": 3, "NO_DEVICE": 4, "HIP": 5, "BUDGET": 6,
+                "NONFINITE": 7}
 
 
 class Grid(C.Structure):
@@ -63,7 +65,9 @@ class RunResult(C.Structure):
 
 
 class ThyError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -93,6 +97,15 @@ def lib():
         L.thy_footprint.restype = C.c_size_t
         L.thy_last_timing.argtypes = [C.c_void_p, _dp]
         L.thy_copy_probe.argtypes = [C.c_size_t, C.c_int, _dp]
+        L.thy_batch_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
+        L.thy_batch_destroy.argtypes = [C.c_void_p]
+        L.thy_batch_last_error.argtypes = [C.c_void_p]
+        L.thy_batch_last_error.restype = C.c_char_p
+        L.thy_batch_add.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), C.POINTER(C.c_int)]
+        L.thy_batch_advance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.thy_batch_status.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(RunResult)]
+        L.thy_batch_remove.argtypes = [C.c_void_p, C.c_int]
+        L.thy_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -217,13 +230,16 @@ class Solver:
         self._chk(lib().thy_run_more(self.h, int(n_steps), C.byref(r)), "run_more")
         return {"steps": r.steps, "t_final": r.t_final, "seconds": r.seconds, "n_samples": r.n_samples}
 
-    def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None, on_step=None):
+    def run_config(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None,
+                   on_step=None, save_interval=None):
+        """thy_run_config of this problem; the objects it points to are kept on self."""
         pr = self.prob
         faults = sorted(pr.faults if faults is None else faults, key=lambda f: f["time"])
         farr = (Fault * max(1, len(faults)))(*[fault_of(f) for f in faults])
         gpos = np.ascontiguousarray(pr.gauge_pos if pr.gauge_pos is not None else np.zeros(0), np.int32)
         c = RunConfig()
-        c.t_end, c.save_interval, c.series_interval = pr.t_end, pr.save_interval, pr.series_interval
+        c.t_end, c.series_interval = pr.t_end, pr.series_interval
+        c.save_interval = pr.save_interval if save_interval is None else save_interval
         c.n_faults, c.faults = len(faults), farr
         c.crop_flag, c.crop_value = pr.crop_flag, pr.crop_value
         c.n_gauges, c.gauge_pos = len(gpos), gpos.ctypes.data_as(_ip)
@@ -232,19 +248,111 @@ class Solver:
         oka = OKADAFN(lambda u, k: on_okada(k)) if on_okada else OKADAFN()
         stp = STEPFN(lambda u, it, dt, t: on_step(it, dt, t)) if on_step else STEPFN()
         c.on_snapshot, c.on_okada, c.on_step = snap, oka, stp
-        r = RunResult()
-        self._cb = (snap, oka, stp, farr, gpos)  # callbacks must outlive a later run_more
-        self._chk(lib().thy_run(self.h, C.byref(c), C.byref(r)), "run")
-        ng = len(gpos)
-        ns = r.n_samples
+        self._cb = (snap, oka, stp, farr, gpos, c)  # must outlive a later run_more / batch advance
+        return c
+
+    def gauges(self, n_samples):
+        ng = len(self._cb[4])
+        ns = n_samples
         bufs = [np.zeros(max(1, ns), np.float32)] + [np.zeros(max(1, ns * ng), np.float32) for _ in range(3)] + \
                [np.zeros(max(1, ng), np.float32) for _ in range(3)]
         self._chk(lib().thy_get_gauges(self.h, *[b.ctypes.data_as(_fp) for b in bufs]), "get_gauges")
-        return {"steps": r.steps, "t_final": r.t_final, "dt_initial": r.dt_initial, "seconds": r.seconds,
-                "n_samples": ns, "n_snapshots": r.n_snapshots, "g_time": bufs[0][:ns],
+        return {"n_samples": ns, "g_time": bufs[0][:ns],
                 "g_eta": bufs[1][:ns * ng].reshape(ns, ng), "g_u": bufs[2][:ns * ng].reshape(ns, ng),
                 "g_v": bufs[3][:ns * ng].reshape(ns, ng), "g_eta_min": bufs[4][:ng], "g_eta_max": bufs[5][:ng],
                 "g_bathy": bufs[6][:ng]}
+
+    def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None, on_step=None):
+        c = self.run_config(max_steps, steps_per_sync, on_snapshot, on_okada, faults, on_step)
+        r = RunResult()
+        self._chk(lib().thy_run(self.h, C.byref(c), C.byref(r)), "run")
+        out = {"steps": r.steps, "t_final": r.t_final, "dt_initial": r.dt_initial, "seconds": r.seconds,
+               "n_snapshots": r.n_snapshots}
+        out.update(self.gauges(r.n_samples))
+        return out
+
+
+SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
+
+
+class Batch:
+    """Many scenarios advanced together (thy_batch_* in include/thysea.h).
+
+    add() admits a Solver (created with upload) with its run configuration; advance()
+    runs one captured step sequence for every running scenario; finished scenarios
+    are read through their Solver and released with remove()."""
+
+    def __init__(self, capacity, hbm_budget=0, steps_per_sync=0):
+        h = C.c_void_p()
+        rc = lib().thy_batch_create(int(capacity), int(hbm_budget), int(steps_per_sync), C.byref(h))
+        self.h = h
+        if rc != 0:
+            raise ThyError(f"thy_batch_create: {STATUS.get(rc, rc)}")
+        self.solvers = {}
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().thy_batch_last_error(self.h).decode(errors="replace")
+            raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {msg}", rc)
+
+    def add(self, solver, steps_per_sync=0):
+        """Admit a scenario; returns its slot, or None when the budget or the slots are full."""
+        c = solver.run_config(steps_per_sync=steps_per_sync, save_interval=-1.0)
+        slot = C.c_int()
+        rc = lib().thy_batch_add(self.h, solver.h, C.byref(c), C.byref(slot))
+        if rc == STATUS_CODES["BUDGET"]:
+            return None
+        self._chk(rc, "batch_add")
+        self.solvers[slot.value] = solver
+        return slot.value
+
+    def advance(self):
+        n = C.c_int()
+        self._chk(lib().thy_batch_advance(self.h, C.byref(n)), "batch_advance")
+        return n.value
+
+    def status(self, slot):
+        st = C.c_int()
+        r = RunResult()
+        self._chk(lib().thy_batch_status(self.h, int(slot), C.byref(st), C.byref(r)), "batch_status")
+        return st.value, {"steps": r.steps, "t_final": r.t_final, "dt_initial": r.dt_initial, "n_samples": r.n_samples}
+
+    def finished(self):
+        return [s for s in self.solvers if self.status(s)[0] == SLOT_FINISHED]
+
+    def result(self, slot):
+        """Max height, arrival times and gauges of a finished scenario."""
+        st, r = self.status(slot)
+        if st != SLOT_FINISHED:
+            raise ThyError(f"slot {slot} is not finished")
+        s = self.solvers[slot]
+        out = dict(r)
+        out["max_height"] = s.field("max_height")
+        out["arrival_times"] = s.field("arrival_times")
+        out.update(s.gauges(r["n_samples"]))
+        return out
+
+    def remove(self, slot):
+        self._chk(lib().thy_batch_remove(self.h, int(slot)), "batch_remove")
+        return self.solvers.pop(slot)
+
+    def info(self):
+        n = C.c_int()
+        b = C.c_size_t()
+        sec = C.c_double()
+        self._chk(lib().thy_batch_info(self.h, C.byref(n), C.byref(b), C.byref(sec)), "batch_info")
+        return {"running": n.value, "bytes": b.value, "gpu_seconds": sec.value}
+
+    def close(self):
+        if self.h:
+            lib().thy_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def copy_probe(nbytes, reps=10):
