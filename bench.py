@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tsunami-hysea_amd"))
 
@@ -49,6 +51,68 @@ def cpu_baseline(prob, steps, threads):
     return prob.nx * prob.ny * steps / sec / 1e9, sec
 
 
+def batched_phase(n, sample, sync):
+    """256 scenarios on one 1000x800 grid: one batch vs the plain solver one at a time
+    (timed on ``sample`` of them, scaled to n).  Batched results of the sampled
+    scenarios are compared with their plain runs."""
+    from thysea import scenario, serving
+    probs = scenario.batch_problems(n)
+    keep = list(range(min(sample, n)))
+    log(f"batched: {n} scenarios on {probs[0].nx}x{probs[0].ny}")
+    t0 = time.perf_counter()
+    rec_b, res_b, gpu_b = serving.run_batched(probs, steps_per_sync=sync, keep=keep)
+    wall_b = time.perf_counter() - t0
+    log(f"batched: {wall_b:.2f} s wall, {gpu_b:.2f} s in step launches")
+    t0 = time.perf_counter()
+    rec_s, res_s = serving.run_sequential([probs[i] for i in keep], steps_per_sync=sync, keep=keep)
+    wall_s = time.perf_counter() - t0
+    per_s = wall_s / len(keep)
+    log(f"sequential: {len(keep)} scenarios in {wall_s:.2f} s ({per_s*1e3:.1f} ms each)")
+    err = 0.0
+    for i in keep:
+        a, b = res_b[i]["max_height"], res_s[i]["max_height"]
+        wet = b > -9000
+        peak = float(np.max(np.abs(b[wet]))) if wet.any() else 1.0
+        err = max(err, float(np.max(np.abs(a[wet] - b[wet]))) / peak)
+        assert res_b[i]["steps"] == res_s[i]["steps"]
+    cells = probs[0].nx * probs[0].ny
+    steps = sum(r["steps"] for r in rec_b)
+    st = serving.latency_stats(rec_b)
+    return {"workload": f"{n} scenarios, shared {probs[0].nx}x{probs[0].ny} grid, Mw 7.0-8.8, 3600 s each",
+            "scenarios_per_hour": n / wall_b * 3600.0,
+            "sequential_scenarios_per_hour": 3600.0 / per_s,
+            "speedup_vs_sequential": (n / wall_b) / (1.0 / per_s),
+            "gcell_steps_per_s": cells * steps / wall_b / 1e9,
+            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "wall_s": wall_b,
+            "sequential_sample": len(keep), "max_height_rel_err_vs_plain": err}
+
+
+def serving_phase(rate, horizon, sync, seq):
+    """Poisson arrivals of requests on grids of different sizes: batched serving, and
+    (when ``seq``) the same requests through the plain solver one at a time."""
+    from thysea import scenario, serving
+    reqs = scenario.serving_requests(rate, horizon)
+    log(f"serving: {len(reqs)} requests over {horizon:.0f} s at {rate}/s")
+    probs = [scenario.request_problem(spec) for _, spec in reqs]
+    arr = [a for a, _ in reqs]
+    keep = list(range(0, len(probs), max(1, len(probs) // 4)))
+    rec_b, res_b, gpu_b = serving.run_batched(probs, arrivals=arr, steps_per_sync=sync, keep=keep)
+    out = {"workload": f"Poisson {rate}/s for {horizon:.0f} s, grids 300x300-2400x1800, 1800-10800 s, Mw 7-9",
+           "batched": serving.latency_stats(rec_b), "gpu_s_in_step_launches": gpu_b,
+           "cell_steps": int(sum(p.nx * p.ny * r["steps"] for p, r in zip(probs, rec_b)))}
+    if seq:
+        rec_s, res_s = serving.run_sequential(probs, arrivals=arr, steps_per_sync=sync, keep=keep)
+        out["sequential"] = serving.latency_stats(rec_s)
+        err = 0.0
+        for i in keep:
+            a, b = res_b[i]["max_height"], res_s[i]["max_height"]
+            wet = b > -9000
+            err = max(err, float(np.max(np.abs(a[wet] - b[wet]))) / max(1e-30, float(np.max(np.abs(b[wet])))))
+        out["max_height_rel_err_vs_plain"] = err
+    log("serving:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +130,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for counter scaling)")
+    ap.add_argument("--batch", type=int, default=256, help="batched scenarios (0: skip)")
+    ap.add_argument("--batch-sample", type=int, default=8, help="of them, also run one at a time")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving arrival window (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--no-serve-sequential", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -122,6 +191,9 @@ def main():
                "sample": f"{args.cpu_steps} steps of the same {prob.nx}x{prob.ny} grid after Okada "
                          f"({sec:.1f} s), FP64 oracle, OpenMP"}
     solver.close()
+    batched = batched_phase(args.batch, args.batch_sample, args.sync_every) if args.batch > 0 else None
+    serve = (serving_phase(args.serve_rate, args.serve_seconds, args.sync_every, not args.no_serve_sequential)
+             if args.serve_seconds > 0 else None)
     out = {
         "metric": METRIC, "value": value, "unit": "Gcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -138,6 +210,10 @@ def main():
                      "bytes_per_cell_step": B_ALG[track]},
         "cpu_baseline": cpu,
     }
+    if batched:
+        out["batched"] = batched
+    if serve:
+        out["serving"] = serve
     if copy:
         out["copy_probe"] = copy
     print(json.dumps(out), flush=True)

@@ -171,6 +171,66 @@ def magnitude_fault(mw, lon_c, lat_c, strike, dip, rake=90.0, mu=3e10, time=0.0)
                 length=L, width=W, strike=strike, dip=dip, rake=rake, slip=slip)
 
 
+def random_faults(lon, lat, n, seed, mw_range=(7.0, 8.8), stream=21):
+    """n magnitude-scaled thrust faults: Mw uniform over ``mw_range`` (log-uniform
+    seismic moment), epicentre in the basin part of the grid, strike uniform, dip in
+    [10, 30] degrees."""
+    u = uniform(seed, 5 * n, stream=stream).reshape(n, 5)
+    out = []
+    for k in range(n):
+        mw = mw_range[0] + (mw_range[1] - mw_range[0]) * u[k, 0]
+        lon_c = float(lon[0] + (0.15 + 0.45 * u[k, 1]) * (lon[-1] - lon[0]))
+        lat_c = float(lat[0] + (0.25 + 0.5 * u[k, 2]) * (lat[-1] - lat[0]))
+        out.append(magnitude_fault(mw, lon_c, lat_c, 360.0 * u[k, 3], 10.0 + 20.0 * u[k, 4]))
+    return out
+
+
+def _in_memory(name, lon, lat, z, faults, t_sim, n_side, series_s):
+    pr = P.build(name, name, lon, lat, z, faults=faults, t_tot=t_sim, t_series=series_s, eps_h=1e-5,
+                 dif_at=1e-6)
+    glon, glat = gauges_lattice(lon, lat, n_side)
+    pr.gauge_pos = np.array([P.nearest_cell(lon, lat, a, b) for a, b in zip(glon, glat)], np.int32)
+    return pr
+
+
+def batch_problems(n, seed=20240601, nx=1000, ny=800, t_sim=3600.0, series_s=60.0):
+    """The batched-scenario workload: n scenarios on one shared 1000x800 grid (east
+    shelf and coast, 10 seamounts), each with its own magnitude-scaled fault
+    (Mw 7.0-8.8) and 16 gauges, 3600 s each."""
+    import dataclasses
+    lon, lat, z = bathymetry(nx, ny, 140.0, 10.0, 10, seed, coast=("east",))
+    faults = random_faults(lon, lat, n, seed)
+    base = _in_memory("batch", lon, lat, z, faults[:1], t_sim, 4, series_s)
+    # the scenarios differ only in their fault: share the grid arrays
+    return [dataclasses.replace(base, faults=[dict(f)]) for f in faults]
+
+
+def serving_requests(rate, horizon_s, seed=20240601, nx_range=(300, 2400), ny_range=(300, 1800),
+                     t_range=(1800.0, 10800.0), mw_range=(7.0, 9.0)):
+    """Poisson arrivals (``rate`` per second over ``horizon_s``) of scenario requests:
+    grid size uniform in the ranges, simulated duration uniform in ``t_range``, one
+    Mw source.  Returns [(arrival_s, spec)], spec = dict(nx, ny, t_sim, seed)."""
+    n_max = int(rate * horizon_s * 3 + 20)
+    u = uniform(seed, 5 * n_max, stream=31).reshape(n_max, 5)
+    out, t = [], 0.0
+    for k in range(n_max):
+        t += -math.log(1.0 - u[k, 0]) / rate
+        if t > horizon_s:
+            break
+        out.append((t, dict(nx=int(nx_range[0] + (nx_range[1] - nx_range[0]) * u[k, 1]),
+                            ny=int(ny_range[0] + (ny_range[1] - ny_range[0]) * u[k, 2]),
+                            t_sim=t_range[0] + (t_range[1] - t_range[0]) * u[k, 3],
+                            seed=int(seed + 1000 + k), mw_range=mw_range)))
+    return out
+
+
+def request_problem(spec, series_s=60.0):
+    """The problem of one serving request (its own grid, one random fault)."""
+    lon, lat, z = bathymetry(spec["nx"], spec["ny"], 140.0, 10.0, 10, spec["seed"], coast=("east",))
+    f = random_faults(lon, lat, 1, spec["seed"], mw_range=spec["mw_range"])
+    return _in_memory("request", lon, lat, z, f, spec["t_sim"], 4, series_s)
+
+
 def make(workload, outdir, seed=20240601, nx=None, ny=None, t_sim=None, gauges=True, borders=None):
     """Write the inputs of a named workload to ``outdir``; return the parameter-file path."""
     os.makedirs(outdir, exist_ok=True)
