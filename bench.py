@@ -113,6 +113,62 @@ def serving_phase(rate, horizon, sync, seq):
     return out
 
 
+def precision_phase():
+    """Precision study on the 2000x2000 grid (7200 s, Mw 8.5): the FP32 production path
+    against the FP64-everywhere run of the same kernels, plus mass drift with walls."""
+    from thysea import native, scenario
+    prob = scenario.problem("precision")
+    log(f"precision: {prob.nx}x{prob.ny}, {prob.t_end * prob.T:.0f} s")
+    res = {}
+    for prec in (0, 1):
+        s = native.Solver(prob, precision=prec)
+        t0 = time.perf_counter()
+        r = s.run(steps_per_sync=48)
+        wall = time.perf_counter() - t0
+        h, H, qx, qy = s.state()
+        res[prec] = dict(r=r, wall=wall, eta=h - H - prob.hmin, maxh=s.field("max_height"),
+                         arr=s.field("arrival_times"))
+        s.close()
+        log(f"precision: fp{32 if prec == 0 else 64} {r['steps']} steps, {wall:.2f} s")
+    a, b = res[0], res[1]
+    wet = b["maxh"] > -9000
+    peak = float(np.max(np.abs(b["maxh"][wet])))
+    eta_peak = float(np.max(np.abs(b["eta"])))
+    both = (a["arr"] >= 0) & (b["arr"] >= 0)
+    darr = np.abs(a["arr"][both] - b["arr"][both])
+    # gauge arrival: first sample with |eta| above 1 cm
+    def gauge_arrival(r):
+        g, t = r["g_eta"], r["g_time"]
+        out = []
+        for k in range(g.shape[1]):
+            hit = np.nonzero(np.abs(g[:, k]) > 0.01)[0]
+            out.append(float(t[hit[0]]) if len(hit) else np.nan)
+        return np.array(out)
+    ga, gb = gauge_arrival(a["r"]), gauge_arrival(b["r"])
+    ok = ~np.isnan(ga) & ~np.isnan(gb)
+    walls = scenario.problem("precision", borders=(-1, -1, -1, -1))
+    drift = {}
+    for prec in (0, 1):
+        s = native.Solver(walls, precision=prec)
+        s.run(max_steps=1, steps_per_sync=3)  # applies the Okada source, one step
+        m0 = s.mass()
+        s.run_more(600)
+        drift["fp32" if prec == 0 else "fp64"] = abs(s.mass() - m0) / m0
+        s.close()
+    out = {"workload": f"{prob.nx}x{prob.ny}, {prob.t_end * prob.T:.0f} s, Mw 8.5 (L 300, W 100 km, slip 8 m)",
+           "steps": [int(a["r"]["steps"]), int(b["r"]["steps"])],
+           "eta_linf_rel_final": float(np.max(np.abs(a["eta"] - b["eta"]))) / eta_peak,
+           "max_height_linf_rel": float(np.max(np.abs(a["maxh"][wet] - b["maxh"][wet]))) / peak,
+           "wet_mask_equal": bool(np.array_equal(a["maxh"] > -9000, wet)),
+           "arrival_time_abs_err_s": {"p50": float(np.percentile(darr, 50)), "p99": float(np.percentile(darr, 99)),
+                                      "max": float(darr.max())},
+           "gauge_arrival_abs_err_s_max": float(np.max(np.abs(ga[ok] - gb[ok]))) if ok.any() else None,
+           "mass_drift_walls_600_steps": drift,
+           "fp64_over_fp32_time": b["wall"] / a["wall"]}
+    log("precision:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +191,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving arrival window (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--no-serve-sequential", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,6 +221,11 @@ def main():
     log("warmup", args.warmup, "steps (includes Okada at t=0)")
     w = solver.run(max_steps=args.warmup, steps_per_sync=args.sync_every)
     log("warmup done: t =", w["t_final"] * prob.T, "s")
+    tuned = None
+    if args.kernel == "sweep" and args.strip_rows == 0:
+        rows, tms = solver.autotune(reps=5)
+        tuned = {"strip_rows": rows, "ms_per_step": tms}
+        log(f"autotune: {rows}-row strips, {tms:.4f} ms per step")
     t0 = time.perf_counter()
     r = solver.run_more(args.steps)
     wall = time.perf_counter() - t0
@@ -194,6 +256,7 @@ def main():
     batched = batched_phase(args.batch, args.batch_sample, args.sync_every) if args.batch > 0 else None
     serve = (serving_phase(args.serve_rate, args.serve_seconds, args.sync_every, not args.no_serve_sequential)
              if args.serve_seconds > 0 else None)
+    prec = None if args.no_precision else precision_phase()
     out = {
         "metric": METRIC, "value": value, "unit": "Gcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -202,7 +265,7 @@ def main():
         "config": {"workload": f"{prob.nx}x{prob.ny} lon/lat grid ({ncell/1e6:.1f}M cells, 2 arc-min), "
                                "4 Okada subfaults, 64 gauges, open borders",
                    "cells": ncell, "track_arrival": track, "steps_per_sync": args.sync_every,
-                   "step_kernel": args.kernel,
+                   "step_kernel": args.kernel, "autotune": tuned,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -212,6 +275,8 @@ def main():
     }
     if batched:
         out["batched"] = batched
+    if prec:
+        out["precision_study"] = prec
     if serve:
         out["serving"] = serve
     if copy:

@@ -176,6 +176,13 @@ size_t thy_footprint(int nx, int ny, const thy_params *params);
    achievable copy bandwidth beside the step kernel. */
 int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
+/* Strip-height autotuner of the row-sweep kernel.  No reference counterpart (the
+   reference has one fixed launch shape).  Times `reps` step launches (HIP events) from
+   the current state for strip heights 16, 24, 32, 40, 48 and 64 rows and keeps the
+   fastest; the state, control blocks and dt slots are left as they were.  Call after
+   thy_upload_state, before or between runs. */
+int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_per_step);
+
 /* Wall time (ms) of the last kernel launch group, for benchmarking. */
 int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);
 

@@ -222,3 +222,76 @@ def test_batch_budget_admission():
     b.close()
     s1.close()
     s2.close()
+
+
+def _fields(s):
+    h, H, qx, qy = s.state()
+    return h - H, qx, qy, s.field("max_height"), s.field("arrival_times")
+
+
+def test_sweep_results_independent_of_strip_height():
+    """Each face is evaluated once with the same arithmetic whatever the strip
+    boundaries, so the state after 25 steps is bit-identical for any strip height."""
+    pr = make_problem(nx=131, ny=97, seamounts=5, seed=11)
+    ref = None
+    for rows in (3, 7, 32, 64):
+        s = _native().Solver(pr, strip_rows=rows)
+        s.run(max_steps=25, steps_per_sync=6)
+        got = _fields(s)
+        s.close()
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b), rows
+
+
+def test_autotune_mid_run_leaves_the_run_unchanged():
+    """thy_autotune between thy_run and thy_run_more picks a strip height and restores
+    the state, control blocks and dt slots: the run ends exactly as without it."""
+    pr = make_problem(nx=131, ny=97, seamounts=5, seed=11)
+    a = _native().Solver(pr)
+    a.run(max_steps=30, steps_per_sync=6)
+    b = _native().Solver(pr)
+    b.run(max_steps=12, steps_per_sync=6)
+    rows, ms = b.autotune(reps=2)
+    assert rows in (16, 24, 32, 40, 48, 64) and ms > 0
+    r = b.run_more(18)
+    assert r["steps"] == 30
+    for x, y in zip(_fields(a), _fields(b)):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+
+
+def test_precision_fp32_against_fp64_run():
+    """Precision study on a small grid: the FP32 production path against the
+    FP64-everywhere run of the same kernels (max height within 1e-4 of its peak,
+    arrival times within one step, same step count within 1), and mass drift with walls
+    below 1e-6 for both."""
+    nat = _native()
+    pr = make_problem(nx=120, ny=100, seamounts=5, seed=4, t_sim=1200.0)
+    out = {}
+    for prec in (0, 1):
+        s = nat.Solver(pr, precision=prec)
+        r = s.run(steps_per_sync=6)
+        out[prec] = (r, s.field("max_height"), s.field("arrival_times"))
+        s.close()
+    (r32, m32, a32), (r64, m64, a64) = out[0], out[1]
+    assert abs(r32["steps"] - r64["steps"]) <= 1
+    wet = m64 > -9000
+    assert np.array_equal(m32 > -9000, wet)
+    assert rel_linf(m32[wet], m64[wet], np.max(np.abs(m64[wet]))) < 1e-4
+    both = (a32 >= 0) & (a64 >= 0)
+    dt_s = r64["t_final"] * pr.T / r64["steps"]
+    assert np.percentile(np.abs(a32[both] - a64[both]), 99) <= 2.01 * dt_s
+    prw = make_problem(nx=96, ny=80, borders=(-1, -1, -1, -1), seed=4)
+    for prec in (0, 1):
+        s = nat.Solver(prw, precision=prec)
+        s.okada(prw.faults[0])
+        m0 = s.mass()
+        dt = s.initial_dt()
+        for _ in range(60):
+            dt = s.step(dt)
+        assert abs(s.mass() - m0) / m0 < 1e-6
+        s.close()

@@ -325,6 +325,14 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
     return THY_OK;
 }
 
+// rebuild the dispatch order from the depth on the device (after a strip-height change)
+template <typename T>
+int rebuild_order_t(thy_ctx *ctx) {
+    std::vector<T> D((size_t)ctx->pitch * ctx->ny);
+    HIPCHK(hipMemcpy(D.data(), ctx->D, D.size() * sizeof(T), hipMemcpyDeviceToHost));
+    return build_order<T>(ctx, D);
+}
+
 template <typename T>
 int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
@@ -763,6 +771,65 @@ int thy_apply_okada(thy_ctx *ctx, const thy_fault *f, int crop_flag, double crop
 int thy_initial_dt(thy_ctx *ctx, double *dt) {
     if (!ctx || !dt) return THY_ERR_INVALID;
     return ctx->prec ? initial_dt_t<double>(ctx, dt) : initial_dt_t<float>(ctx, dt);
+}
+
+int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
+    if (!ctx) return THY_ERR_INVALID;
+    if (ctx->kernel != 0) return fail(ctx, THY_ERR_INVALID, "autotune covers the row-sweep kernel");
+    if (ctx->order_h.empty()) return fail(ctx, THY_ERR_INVALID, "autotune needs an uploaded state");
+    if (reps <= 0) reps = 5;
+    int r;
+    // the run's control blocks and dt slots are restored afterwards
+    Ctrl saved[3];
+    unsigned int saved_dt[3];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(saved, ctx->ctrl, sizeof(saved), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(saved_dt, ctx->dtmin, sizeof(saved_dt), hipMemcpyDeviceToHost));
+    const int saved_slot = ctx->slot;
+    double dt;
+    if ((r = thy_initial_dt(ctx, &dt)) != THY_OK) return r;
+    Ctrl c = idle_ctrl(ctx);
+    c.t = saved[saved_slot].t;
+    c.dt = dt;
+    c.dt_valid = 1;
+    c.cur = ctx->cur;
+    const int cand[] = {16, 24, 32, 40, 48, 64};
+    int best = ctx->srows;
+    double best_ms = 1e30;
+    for (int S : cand) {
+        if (S > ctx->ny && S != cand[0]) continue;
+        ctx->srows = S;
+        r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
+        if (r != THY_OK) return r;
+        std::vector<float> t;
+        for (int k = 0; k <= reps; k++) {  // the first launch is a warm-up
+            ctx->slot = 0;
+            if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            hipError_t e = launch_step(ctx);
+            if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("autotune launch: ") + hipGetErrorString(e));
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+            HIPCHK(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (k > 0) t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        const double med = t[t.size() / 2];
+        if (med < best_ms) {
+            best_ms = med;
+            best = S;
+        }
+    }
+    ctx->srows = best;
+    r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
+    if (r != THY_OK) return r;
+    ctx->slot = saved_slot;
+    HIPCHK(hipMemcpy(ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
+    if (strip_rows) *strip_rows = best;
+    if (ms_best) *ms_best = best_ms;
+    return THY_OK;
 }
 
 int thy_step(thy_ctx *ctx, double dt, double *dt_next) {

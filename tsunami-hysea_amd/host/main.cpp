@@ -220,6 +220,14 @@ int main(int argc, char *argv[]) {
         thy_destroy(ctx);
         return EXIT_FAILURE;
     }
+    // strip height of the row-sweep kernel chosen by measurement (THYSEA_AUTOTUNE=0 keeps
+    // the default); the tuner leaves the state as it was
+    if (precision == 0 && env_int("THYSEA_AUTOTUNE", 1)) {
+        int rows = 0;
+        double ms = 0.0;
+        if (thy_autotune(ctx, 3, &rows, &ms) != THY_OK)
+            std::cerr << "Warning: autotune failed: " << thy_last_error(ctx) << std::endl;
+    }
     RunState rs;
     rs.p = &p;
     rs.ctx = ctx;

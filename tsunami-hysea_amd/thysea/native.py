@@ -97,6 +97,7 @@ def lib():
         L.thy_footprint.restype = C.c_size_t
         L.thy_last_timing.argtypes = [C.c_void_p, _dp]
         L.thy_copy_probe.argtypes = [C.c_size_t, C.c_int, _dp]
+        L.thy_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
         L.thy_batch_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
         L.thy_batch_destroy.argtypes = [C.c_void_p]
         L.thy_batch_last_error.argtypes = [C.c_void_p]
@@ -223,6 +224,14 @@ class Solver:
         v = C.c_double()
         self._chk(lib().thy_mass(self.h, C.byref(v)), "mass")
         return v.value
+
+    def autotune(self, reps=5):
+        """Pick the fastest strip height of the row-sweep kernel (thy_autotune);
+        returns (strip_rows, ms per step)."""
+        rows = C.c_int()
+        ms = C.c_double()
+        self._chk(lib().thy_autotune(self.h, int(reps), C.byref(rows), C.byref(ms)), "autotune")
+        return rows.value, ms.value
 
     def run_more(self, n_steps):
         """Continue the last run for n_steps more steps; returns (steps, t, gpu_seconds)."""
