@@ -264,9 +264,12 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
             const T h0 = i0.e + i0.D;
             const bool din0 = __all(h0 > hpos) != 0;
 
+            // Stages run only for the rows and faces that some output row of the strip
+            // depends on (h* rows y0-2 .. y1+1, momentum faces of rows y0 .. y1-1 and
+            // y faces y0 .. y1); the halo iterations skip the rest.
             // ---- mass x-faces of row r ----
-            T m0;
-            {
+            T m0 = T(0);
+            if (r >= y0 - 2 && r <= y1 + 1) {
                 const RowGeo<T> rg = uniform_load(a.row + clamp_row(r));
                 if (din0 && !seg_ac) {
                     const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
@@ -282,7 +285,7 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
                 }
             }
             // ---- mass y-face g = r-1 (rows r-3..r) ----
-            {
+            if (r - 1 >= y0 - 2 && r - 1 <= y1 + 2) {
                 const int g = r - 1;
                 const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
                 const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
@@ -301,15 +304,18 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
             // ---- h* (as e*) of row r-2, with the sponge ----
             T s2;
             const RowGeo<T> rg2 = uniform_load(a.row + clamp_row(r - 2));
-            {
+            const bool hrow = r - 2 >= y0 - 2 && r - 2 <= y1 + 1;  // h* of row r-2 is needed
+            if (hrow) {
                 T es = i2.e + dt * rg2.invA * m2;
                 if (!(es + i2.D > T(0))) es = -i2.D;
                 if (any_sponge) es = sponge_eta(a, es, i2.D, isp, r - 2, rg2.dxs, rg2.dys);
                 if (xedge) es = __shfl(es, src, 64);
                 s2 = es;
+            } else {
+                s2 = i2.e;
             }
             const T hs2 = s2 + i2.D;
-            const bool ds2 = __all(hs2 > hpos) != 0;
+            const bool ds2 = hrow && __all(hs2 > hpos) != 0;
             T sq2 = T(0), u2 = T(0), v2 = T(0);
             if (ds2) {
                 sq2 = Num<T>::sqrt(hs2);
@@ -318,8 +324,10 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
                 v2 = i2.qy * ih;
             }
             // ---- momentum x-faces of row r-2 ----
-            T X2, Y2, W2;
-            if (ds2 && !seg_ac) {
+            T X2 = T(0), Y2 = T(0), W2 = T(0);
+            if (r - 2 < y0 || r - 2 >= y1) {
+                // halo row: no output row uses its x faces
+            } else if (ds2 && !seg_ac) {
                 DC<T> A, B;
                 A.h = from_left(hs2); A.e = from_left(s2); A.D = from_left(i2.D);
                 A.qn = from_left(i2.qx); A.qt = from_left(i2.qy); A.s = from_left(sq2); A.un = from_left(u2);
@@ -340,7 +348,7 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
                 W2 = f.w + from_right(f.w);
             }
             // ---- momentum y-face g = r-3 (rows r-5..r-2) ----
-            {
+            if (r - 3 >= y0 && r - 3 <= y1) {
                 const int g = r - 3;
                 const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
                 const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
