@@ -61,7 +61,9 @@ typedef struct thy_params {
     int precision;     /* 0: FP32 state and fluxes; 1: FP64 everywhere */
     int track_arrival; /* 1: arrival times tracked (reference behaviour) */
     int tile_x, tile_y;/* 2D-tile kernel: tile shape, 0: default */
-    int kernel;        /* step kernel: 0 row sweep (default), 1 2D LDS tiles */
+    int kernel;        /* step kernel: 0 row sweep with a deep-water launch for blocks far from
+                          coast and borders (default), 1 2D LDS tiles, 2 row sweep with the
+                          general faces everywhere */
     int strip_rows;    /* row-sweep kernel: rows per strip, 0: default */
 } thy_params;
 

@@ -295,3 +295,53 @@ def test_precision_fp32_against_fp64_run():
             dt = s.step(dt)
         assert abs(s.mass() - m0) / m0 < 1e-6
         s.close()
+
+
+def _split_pair(pr, steps):
+    out = []
+    for kernel in ("sweep", "sweep_general"):
+        s = _native().Solver(pr, kernel=kernel, strip_rows=16)
+        s.run(max_steps=steps, steps_per_sync=6)
+        out.append(_fields(s))
+        s.close()
+    return out
+
+
+def test_deep_split_identical_to_general_faces():
+    """The deep-water launch (interior blocks far from coast and borders) and the
+    general launch evaluate the deep faces with the same functions: the split run is
+    bit-identical to the general-faces-only run."""
+    pr = make_problem(nx=400, ny=300, seamounts=6, seed=21)
+    a, b = _split_pair(pr, 30)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_deep_split_hands_back_failing_blocks():
+    """A dry patch carved into deep water (classified deep from the still-water depth,
+    but h <= hpos at run time): the deep-water launch hands those blocks back to the
+    general launch of the same step; the result is still bit-identical to the
+    general-faces-only run and within tolerance of the oracle."""
+    pr = make_problem(nx=400, ny=300, seamounts=0, seed=22, fault=False)
+    H = pr.H.reshape(pr.ny, pr.nx)
+    h = pr.h.reshape(pr.ny, pr.nx).copy()
+    assert (H[140:150, 180:190] > 1.0).all()  # deep water (nondimensional depth > 1000 m)
+    h[140:150, 180:190] = 0.0                 # dry hole: the ocean rushes in (WAF, wet/dry)
+    pr.h = h.ravel() if pr.h.ndim == 1 else h
+    a, b = _split_pair(pr, 20)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    s = _native().Solver(pr)
+    o = _oracle(pr)
+    dt = o.initial_dt()
+    for _ in range(20):
+        dn = o.step(dt)
+        s.step(dt)
+        dt = dn
+    hg, Hg, qxg, qyg = s.state()
+    ho, Ho, qxo, qyo = o.state()
+    peak = np.max(np.abs(qxo)) + np.max(np.abs(qyo))
+    assert rel_linf(qxg, qxo, peak) < 1e-4 and rel_linf(qyg, qyo, peak) < 1e-4
+    wet = ho > pr.eps_h
+    assert rel_linf(hg[wet], ho[wet], np.max(np.abs((ho - Ho)[wet]))) < 1e-3
+    s.close()

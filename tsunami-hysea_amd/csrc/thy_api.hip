@@ -48,16 +48,23 @@ struct thy_ctx {
     double *mass_partial = nullptr;
     float *outbuf = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;          // general launch beside the deep-water launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int slot = 0;
     int cur = 0;                 // host mirror of ctrl.cur after a sync
     int tx = 64, ty = 16;
     int chunk_rows = 1;          // tile (strip) rows per XCD chunk of the step kernel's order
-    int kernel = 0;              // 0 row sweep, 1 2D tiles
+    int kernel = 0;              // 0 row sweep (deep-water + general split), 1 2D tiles, 2 row sweep general only
     int srows = 32;              // row sweep: rows per strip
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
     int order_n = 0;
-    std::vector<int> order_h, cls_h;  // host copies: the order and each block's cost class
+    std::vector<int> order_h, cls_h;  // host copies: every block costliest first, and the classes
+    int n_order = 0;             // blocks in `order` (the general launch's share)
+    int *deep = nullptr;         // deep-water blocks (n_deep of them)
+    int n_deep = 0, list_cap = 0;
+    unsigned int *fb_count = nullptr;  // 3 slots
+    int *fb_list = nullptr;            // [3][list_cap]
     hipGraphExec_t gexec = nullptr;
     int gk = 0;
     double last_ms = 0.0;
@@ -114,12 +121,17 @@ StepArgs<T> make_args(thy_ctx *c) {
     const thy_params &p = c->prm;
     a.nx = c->nx; a.ny = c->ny; a.pitch = c->pitch;
     a.periodic = (std::fabs(p.borde_izq - 2.0) < 2.220446049250313e-16 && std::fabs(p.borde_der - 2.0) < 2.220446049250313e-16) ? 1 : 0;
-    if (c->kernel == 0) {
+    if (c->kernel == 0 || c->kernel == 2) {
         const int nseg = (c->nx + SW_OUT - 1) / SW_OUT;
         a.tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES;
         a.tiles_y = (c->ny + c->srows - 1) / c->srows;
         a.srows = c->srows;
         a.order = c->order;
+        a.n_order = c->n_order;
+        a.deep = c->n_deep > 0 ? c->deep : nullptr;
+        a.n_deep = c->n_deep;
+        a.fb_count = c->fb_count;
+        a.fb_list = c->fb_list;
     } else {
         a.tiles_x = (c->nx + c->tx - 1) / c->tx;
         a.tiles_y = (c->ny + c->ty - 1) / c->ty;
@@ -158,9 +170,28 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
     StepArgs<T> a = make_args<T>(c);
     const int n_chunks = (a.tiles_y + c->chunk_rows - 1) / c->chunk_rows;
     dim3 grid(((n_chunks + 7) / 8) * 8 * a.chunk), block(256);
-    if (c->kernel == 0) {
-        if (a.order) grid = dim3(a.n_tiles);
-        hipLaunchKernelGGL((k_sweep<T>), grid, dim3(64 * SW_WAVES), 0, c->stream, a, slot);
+    if (c->kernel == 0 || c->kernel == 2) {
+        if constexpr (sizeof(T) == 4) {
+            if (a.deep) {
+                // deep-water blocks on the main stream and the general blocks on the side
+                // stream run side by side; after the join, the hand-backs of this step
+                hipError_t e;
+                if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(c->side, c->ev_fork, 0)) != hipSuccess) return e;
+                if (a.n_order > 0) {
+                    hipLaunchKernelGGL((k_sweep<T>), dim3(a.n_order), dim3(64 * SW_WAVES), 0, c->side, a, slot, 1);
+                    if ((e = hipGetLastError()) != hipSuccess) return e;
+                }
+                hipLaunchKernelGGL(k_sweep_deep, dim3(a.n_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+                if ((e = hipEventRecord(c->ev_join, c->side)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+                hipLaunchKernelGGL((k_sweep<T>), dim3(a.n_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot, 2);
+                return hipGetLastError();
+            }
+        }
+        if (a.order) grid = dim3(std::max(1, a.n_order));
+        hipLaunchKernelGGL((k_sweep<T>), grid, dim3(64 * SW_WAVES), 0, c->stream, a, slot, 0);
         return hipGetLastError();
     }
 #define THY_TILE(TX_, TY_)                                                                            \
@@ -239,7 +270,7 @@ int pick_tile(thy_ctx *c) {
     if (!ok) return fail(c, THY_ERR_INVALID, "unsupported tile shape " + std::to_string(tx) + "x" + std::to_string(ty));
     c->tx = tx;
     c->ty = ty;
-    if (c->prm.kernel != 0 && c->prm.kernel != 1)
+    if (c->prm.kernel < 0 || c->prm.kernel > 2)
         return fail(c, THY_ERR_INVALID, "unknown step kernel " + std::to_string(c->prm.kernel));
     c->kernel = c->prm.kernel;
     if (c->prm.strip_rows < 0 || c->prm.strip_rows > 65536)
@@ -263,7 +294,7 @@ int pick_tile(thy_ctx *c) {
 template <typename T>
 int build_order(thy_ctx *ctx, const std::vector<T> &D) {
     thy_ctx *c = ctx;
-    if (c->kernel != 0) return THY_OK;
+    if (c->kernel == 1) return THY_OK;
     const int nx = c->nx, ny = c->ny, P = c->pitch, S = c->srows;
     const int nseg = (nx + SW_OUT - 1) / SW_OUT;
     const int tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES, tiles_y = (ny + S - 1) / S;
@@ -280,19 +311,34 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
         }
     }
     std::vector<int> cls(n_tiles, 0);
+    std::vector<unsigned char> deep_ok(n_tiles, 0);
+    const bool split = c->kernel == 0 && c->prec == 0;
+    const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
     for (int t = 0; t < n_tiles; t++) {
         const int ty = t / tiles_x, tx = t - ty * tiles_x;
         const int y0 = ty * S, y1 = std::min(y0 + S, ny);
         const int rows = (y1 - y0) + 8;
         double worst = 0.0;
+        // deep-water launch: rows y0-4 .. y1+3 and each strip's 64 columns clear of the
+        // grid edges (no clamping, no first-order band), of the sponges, and of coast
+        bool dk = split && y0 - 4 >= 2 && y1 + 3 <= ny - 3 && y0 - 2 >= sps && y1 + 1 < ny - spn;
         for (int w = 0; w < SW_WAVES; w++) {
             const int sgi = tx * SW_WAVES + w;
-            if (sgi >= nseg) break;
+            if (sgi >= nseg) {
+                dk = false;
+                break;
+            }
+            const int x0 = sgi * SW_OUT;
+            if (!(x0 - SW_HALO >= 2 && x0 + SW_OUT + SW_HALO - 1 <= nx - 3 && x0 - SW_HALO >= spi &&
+                  x0 + SW_OUT + SW_HALO - 1 < nx - spd))
+                dk = false;
             int nc = 0;
             for (int j = std::max(0, y0 - 4); j < std::min(ny, y1 + 4); j++) nc += coastal[(size_t)j * nseg + sgi];
+            if (nc) dk = false;
             worst = std::max(worst, 1.5 * nc / rows);
         }
         cls[t] = (int)(worst * 4.0 + 0.5);
+        deep_ok[t] = dk ? 1 : 0;
     }
     // chunked XCD order as the base sequence (the kernel's order without a table)
     std::vector<int> base;
@@ -308,20 +354,29 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
     std::stable_sort(base.begin(), base.end(), [&](int a, int b) { return cls[a] > cls[b]; });
     c->order_h = base;
     c->cls_h = cls;
-    if (c->gexec) {  // a captured step sequence holds the old table pointer and grid
+    std::vector<int> gen, deep;
+    for (int t : base) (deep_ok[t] ? deep : gen).push_back(t);
+    if (c->gexec) {  // a captured step sequence holds the old tables and grid sizes
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
         c->gk = 0;
     }
-    if (c->order && c->order_n != n_tiles) {
-        HIPCHK(hipFree(c->order));
-        c->order = nullptr;
-    }
-    if (!c->order) {
+    if (c->list_cap < n_tiles) {
+        void *ptrs[] = {c->order, c->deep, c->fb_list};
+        for (void *p : ptrs)
+            if (p) HIPCHK(hipFree(p));
+        c->order = c->deep = c->fb_list = nullptr;
         HIPCHK(hipMalloc(&c->order, (size_t)n_tiles * sizeof(int)));
-        c->order_n = n_tiles;
+        HIPCHK(hipMalloc(&c->deep, (size_t)n_tiles * sizeof(int)));
+        HIPCHK(hipMalloc(&c->fb_list, (size_t)3 * n_tiles * sizeof(int)));
+        c->list_cap = n_tiles;
     }
-    HIPCHK(hipMemcpy(c->order, base.data(), (size_t)n_tiles * sizeof(int), hipMemcpyHostToDevice));
+    if (!c->fb_count) HIPCHK(hipMalloc(&c->fb_count, 3 * sizeof(unsigned int)));
+    HIPCHK(hipMemset(c->fb_count, 0, 3 * sizeof(unsigned int)));
+    c->n_order = (int)gen.size();
+    c->n_deep = (int)deep.size();
+    if (!gen.empty()) HIPCHK(hipMemcpy(c->order, gen.data(), gen.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!deep.empty()) HIPCHK(hipMemcpy(c->deep, deep.data(), deep.size() * sizeof(int), hipMemcpyHostToDevice));
     return THY_OK;
 }
 
@@ -401,6 +456,7 @@ int write_ctrl(thy_ctx *ctx, const Ctrl &c) {
 int reset_dt_slots(thy_ctx *ctx) {
     unsigned int v[3] = {0x7f7fffffu, 0x7f7fffffu, 0x7f7fffffu};
     HIPCHK(hipMemcpy(ctx->dtmin, v, sizeof(v), hipMemcpyHostToDevice));
+    if (ctx->fb_count) HIPCHK(hipMemset(ctx->fb_count, 0, 3 * sizeof(unsigned int)));
     return THY_OK;
 }
 
@@ -666,8 +722,11 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     if (r != THY_OK) return r;
     HIPCHK(hipSetDevice(0));
     HIPCHK(hipStreamCreate(&ctx->stream));
+    HIPCHK(hipStreamCreate(&ctx->side));
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
     for (int k = 0; k < 2; k++) {
         HIPCHK(hipMalloc(&ctx->e[k], b));
@@ -731,11 +790,14 @@ void thy_destroy(thy_ctx *ctx) {
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
-                    ctx->steplog, ctx->order};
+                    ctx->steplog, ctx->order, ctx->deep, ctx->fb_count, ctx->fb_list};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -775,7 +837,7 @@ int thy_initial_dt(thy_ctx *ctx, double *dt) {
 
 int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
     if (!ctx) return THY_ERR_INVALID;
-    if (ctx->kernel != 0) return fail(ctx, THY_ERR_INVALID, "autotune covers the row-sweep kernel");
+    if (ctx->kernel == 1) return fail(ctx, THY_ERR_INVALID, "autotune covers the row-sweep kernel");
     if (ctx->order_h.empty()) return fail(ctx, THY_ERR_INVALID, "autotune needs an uploaded state");
     if (reps <= 0) reps = 5;
     int r;
@@ -805,6 +867,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
         for (int k = 0; k <= reps; k++) {  // the first launch is a warm-up
             ctx->slot = 0;
             if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
+            if (ctx->fb_count) HIPCHK(hipMemset(ctx->fb_count, 0, 3 * sizeof(unsigned int)));
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
             hipError_t e = launch_step(ctx);
             if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("autotune launch: ") + hipGetErrorString(e));
@@ -843,6 +906,7 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
     unsigned int big = 0x7f7fffffu;
     const int s = ctx->slot;
     HIPCHK(hipMemcpy(&ctx->dtmin[s], &big, sizeof(big), hipMemcpyHostToDevice));
+    if (ctx->fb_count) HIPCHK(hipMemset(&ctx->fb_count[s], 0, sizeof(unsigned int)));
     hipError_t e = launch_step(ctx);
     if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1264,7 +1328,7 @@ const char *thy_batch_last_error(thy_batch *b) { return b ? b->err.c_str() : "nu
 
 int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot_out) {
     if (!b || !ctx || !cfg || !slot_out) return THY_ERR_INVALID;
-    if (ctx->prec != 0 || ctx->kernel != 0)
+    if (ctx->prec != 0 || ctx->kernel == 1)
         return bfail(b, THY_ERR_INVALID, "batched scenarios run the FP32 row-sweep kernel only");
     if (cfg->save_interval >= 0.0) return bfail(b, THY_ERR_INVALID, "batched scenarios take no snapshots");
     if (ctx->order_h.empty()) return bfail(b, THY_ERR_INVALID, "scenario state not uploaded");

@@ -56,7 +56,12 @@ template <typename T> struct StepArgs {
     int nx, ny, pitch, periodic;
     int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
     int srows;                             // row-sweep kernel: rows per strip
-    const int *order;                      // row-sweep kernel: workgroup -> block (or null)
+    const int *order;                      // row-sweep kernel: its blocks, costliest first (or null)
+    int n_order;                           //   number of them
+    const int *deep;                       // deep-water sweep: its blocks (null: no split)
+    int n_deep;
+    unsigned int *fb_count;                // 3 slots: blocks the deep-water sweep hands back
+    int *fb_list;                          // [3][n_deep]
     int sp_sup, sp_inf, sp_izq, sp_der;
     int track_arrival, n_gauges;
     T bsup, binf, bizq, bder;

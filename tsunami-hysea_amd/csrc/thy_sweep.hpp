@@ -432,27 +432,244 @@ __device__ __forceinline__ void block_dt_min(const StepArgs<T> &a, const int slo
     }
 }
 
-// One step of one grid: one workgroup per strip block.
+// One step of one grid, general faces, one workgroup per strip block.  mode 0: every
+// block (the table's order, costliest first, or without a table the chunked XCD
+// order); mode 1: the table's blocks only, beside a deep-water launch; mode 2: the
+// blocks the deep-water launch of this step handed back (the grid has room for all of
+// them; surplus workgroups return at once).
 template <typename T>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot) {
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot, const int mode) {
     __shared__ BlockCtl<T> bc;
     __shared__ float s_red[SW_WAVES];
     const int tid = threadIdx.x;
-    // workgroup order: the host's cost-ordered table when present, else the XCD-aware
-    // chunked order of k_step (chunks of whole strip rows)
     const int bid = blockIdx.x;
+    // the step's writer is the deep-water launch when there is one, else workgroup 0
+    const bool writer = bid == 0 && mode == 0;
+    if (tid == 0) begin_step(a, slot, writer, bc);
+    __syncthreads();
+    if (!bc.run) return;
     int tile;
-    if (a.order) {
-        tile = a.order[bid];
-    } else {
+    if (!a.order) {
         const int loc = bid >> 3, ch = loc / a.chunk;
         tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
+        if (tile >= a.n_tiles) return;
+    } else if (mode != 2) {
+        if (bid >= a.n_order) return;
+        tile = a.order[bid];
+    } else {
+        const int n_fb = min((int)__builtin_amdgcn_readfirstlane(a.fb_count[slot]), a.n_deep);
+        if (bid >= n_fb) return;
+        tile = a.fb_list[slot * a.n_deep + bid];
     }
-    if (tid == 0) begin_step(a, slot, bid == 0, bc);
-    __syncthreads();
-    if (!bc.run || tile >= a.n_tiles) return;
-    const float dtmin = sweep_block(a, bc, tile, bid == 0);
+    const float dtmin = sweep_block(a, bc, __builtin_amdgcn_readfirstlane(tile), writer);
     block_dt_min(a, slot, dtmin, s_red);
+}
+
+// ---------------------------------------------------------------------------
+// Deep-water sweep.  Blocks whose strips and halos lie clear of the borders, the
+// first-order bands and the sponges, and whose still-water depth exceeds 3 hpos, run
+// this lean form of the pipeline: only the 2s faces, no sponge and no near-dry update,
+// no clamped columns or rows, and fewer pipeline registers.  Every wavefront checks at
+// run time the conditions under which the general faces take the 2s branch (every
+// loaded row deeper than hpos; every h* row that a face's stencil reaches deeper than
+// hpos).  A workgroup where a check fails commits no dt and hands its block to the
+// general launch of the same step, which recomputes and rewrites all of it; the two
+// launches evaluate the deep faces with the same functions, so the split changes no
+// result.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool deep_block(const StepArgs<T> &a, const BlockCtl<T> &bc, const int tile,
+                                           float &dtmin_out) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool c1 = bc.cur != 0;
+    const T *__restrict__ ge = c1 ? a.e[1] : a.e[0];
+    const T *__restrict__ gqx = c1 ? a.qx[1] : a.qx[0];
+    const T *__restrict__ gqy = c1 ? a.qy[1] : a.qy[0];
+    const T *__restrict__ gD = a.D;
+    T *__restrict__ oe = c1 ? a.e[0] : a.e[1];
+    T *__restrict__ oqx = c1 ? a.qx[0] : a.qx[1];
+    T *__restrict__ oqy = c1 ? a.qy[0] : a.qy[1];
+    const T dt = (T)bc.dt;
+    const T tnow = (T)bc.t;
+    const int nx = a.nx, P = a.pitch;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int x0 = (txi * SW_WAVES + wv) * SW_OUT;
+    const int y0 = tyi * a.srows;
+    const int y1 = min(y0 + a.srows, a.ny);
+    float dtmin = 3.0e38f;
+    bool ok = true;
+    if (x0 < nx) {
+        // the block classification keeps columns x0-4 .. x0+59 and rows y0-4 .. y1+3
+        // inside the grid: no clamping
+        const unsigned gx = (unsigned)(x0 - SW_HALO + lane);
+        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT;
+        const T hpos = a.hpos, cd2s = T(0.5) * (T(1) - a.cvis);
+        const T *__restrict__ gtar = a.track_arrival ? a.tarr : a.emax;
+        const T *__restrict__ gini = a.track_arrival ? a.eini : a.emax;
+        auto load_row = [&](int r) {
+            const int64_t o = (int64_t)r * P;
+            RowCell<T> c;
+            c.e = (ge + o)[gx];
+            c.D = (gD + o)[gx];
+            c.qx = (gqx + o)[gx];
+            c.qy = (gqy + o)[gx];
+            return c;
+        };
+        auto load_metrics = [&](int jn, T &m, T &t, T &i) {
+            const int64_t o = (int64_t)(jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P;
+            m = (a.emax + o)[gx];
+            t = (gtar + o)[gx];
+            i = (gini + o)[gx];
+        };
+        // pipeline registers: input rows r-4..r, e* rows r-4..r-3 with sqrt(h*) and qy/h*,
+        // mass accumulators rows r-2..r-1, momentum accumulators rows r-4..r-3
+        RowCell<T> i4, i3, i2, i1, i0;
+        i4 = i3 = i2 = i1 = i0 = RowCell<T>{T(0), T(0), T(0), T(0)};
+        T s4 = T(0), s3 = T(0), sq4 = T(0), sq3 = T(0), v4 = T(0), v3 = T(0);
+        T m2 = T(0), m1 = T(0);
+        T X4 = T(0), Y4 = T(0), W4 = T(0), X3 = T(0), Y3 = T(0), W3 = T(0);
+        const int rbeg = y0 - 4, rend = y1 + 3;
+        RowCell<T> buf = load_row(rbeg);
+        T pm, pt, pi;
+        load_metrics(rbeg - 4, pm, pt, pi);
+        for (int r = rbeg; r <= rend; r++) {
+            i0 = buf;
+            buf = load_row(r < rend ? r + 1 : r);
+            const int ju = r - 4;
+            const T cmax = pm, ctar = pt, cini = pi;
+            load_metrics(ju + 1, pm, pt, pi);
+            const T h0 = i0.e + i0.D;
+            if (!__all(h0 > hpos)) {
+                ok = false;
+                break;
+            }
+            // mass x-faces of row r (h* rows y0-2 .. y1+1)
+            T m0 = T(0);
+            if (r >= y0 - 2 && r <= y1 + 1) {
+                const RowGeo<T> rg = uniform_load(a.row + r);
+                const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
+                m0 = F - from_right(F);
+            }
+            // mass y-face g = r-1 (faces y0-2 .. y1+2)
+            if (r - 1 >= y0 - 2 && r - 1 <= y1 + 2) {
+                const FaceGeoY<T> fg = uniform_load(a.fy + (r - 1));
+                const T F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
+                m2 -= F * fg.ic0;
+                m1 += F * fg.ic1;
+            }
+            // h* of row r-2
+            T s2 = T(0), hs2 = T(0), sq2 = T(0), u2 = T(0), v2 = T(0);
+            const bool hrow = r - 2 >= y0 - 2 && r - 2 <= y1 + 1;
+            const RowGeo<T> rg2 = uniform_load(a.row + (r - 2 < y0 - 4 ? y0 - 4 : r - 2));
+            if (hrow) {
+                T es = i2.e + dt * rg2.invA * m2;
+                if (!(es + i2.D > T(0))) es = -i2.D;
+                s2 = es;
+                hs2 = es + i2.D;
+                if (!__all(hs2 > hpos)) {
+                    ok = false;
+                    break;
+                }
+                sq2 = Num<T>::sqrt(hs2);
+                const T ih = Num<T>::rcp(hs2);
+                u2 = i2.qx * ih;
+                v2 = i2.qy * ih;
+            }
+            // momentum x-faces of row r-2 (rows y0 .. y1-1)
+            T X2 = T(0), Y2 = T(0), W2 = T(0);
+            if (r - 2 >= y0 && r - 2 < y1) {
+                DC<T> A, B;
+                A.h = from_left(hs2); A.e = from_left(s2); A.D = from_left(i2.D);
+                A.qn = from_left(i2.qx); A.qt = from_left(i2.qy); A.s = from_left(sq2); A.un = from_left(u2);
+                B.h = hs2; B.e = s2; B.D = i2.D; B.qn = i2.qx; B.qt = i2.qy; B.s = sq2; B.un = u2;
+                const MF<T> f = fast_mom_x(A, B, rg2.lenx, rg2.gx, rg2.valx, dt, cd2s);
+                X2 = -f.pn - from_right(f.mn);
+                Y2 = -f.pt - from_right(f.mt);
+                W2 = f.w + from_right(f.w);
+            }
+            // momentum y-face g = r-3 (faces y0 .. y1)
+            if (r - 3 >= y0 && r - 3 <= y1) {
+                const FaceGeoY<T> fg = uniform_load(a.fy + (r - 3));
+                DC<T> A, B;
+                A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
+                B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
+                const MF<T> f = fast_mom_y(A, B, fg, dt, cd2s);
+                Y4 -= f.mn;
+                X4 -= f.mt;
+                W4 += f.w;
+                Y3 -= f.pn;
+                X3 -= f.pt;
+                W3 += f.w;
+            }
+            // cell update of row r-4 (src/GPU/Volumen_kernel.cu:160-221; h* > hpos here)
+            if (ju >= y0 && ju < y1 && out_col) {
+                const RowGeo<T> rg = uniform_load(a.row + ju);
+                const int64_t go = (int64_t)ju * P + gx;
+                const T D = i4.D, eo = i4.e;
+                if (eo > cmax && eo + D > a.eps_h) a.emax[go] = eo;
+                if (a.track_arrival && ctar < T(0) && tabs(eo - cini) > a.dif_at) a.tarr[go] = tnow;
+                const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
+                dtmin = fminf(dtmin, (float)dtc);
+                const T val = dt * rg.invA;
+                oe[go] = s4;
+                oqx[go] = i4.qx + val * X4;
+                oqy[go] = i4.qy + val * Y4;
+            }
+            i4 = i3; i3 = i2; i2 = i1; i1 = i0;
+            s4 = s3; s3 = s2;
+            sq4 = sq3; sq3 = sq2; v4 = v3; v3 = v2;
+            m2 = m1; m1 = m0;
+            X4 = X3; Y4 = Y3; W4 = W3;
+            X3 = X2; Y3 = Y2; W3 = W2;
+        }
+    }
+    dtmin_out = dtmin;
+    return ok;
+}
+
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<float> a, const int slot) {
+    __shared__ BlockCtl<float> bc;
+    __shared__ float s_red[SW_WAVES];
+    __shared__ int s_ok[SW_WAVES];
+    const int tid = threadIdx.x, bid = blockIdx.x;
+    if (tid == 0) {
+        begin_step(a, slot, bid == 0, bc);  // this launch is the step's writer
+        if (bid == 0) a.fb_count[(slot + 1) % 3] = 0u;  // the next launch's hand-back list
+    }
+    __syncthreads();
+    if (!bc.run) return;
+    if (bid == 0) {
+        const bool c1 = bc.cur != 0;
+        gather_gauges(a, bc, c1 ? a.e[1] : a.e[0], c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], tid,
+                      64 * SW_WAVES);
+    }
+    if (bid >= a.n_deep) return;
+    const int tile = a.deep[bid];
+    float dtmin;
+    const bool ok = deep_block(a, bc, tile, dtmin);
+    dtmin = wave_min(dtmin);
+    if ((tid & 63) == 0) {
+        s_red[tid >> 6] = dtmin;
+        s_ok[tid >> 6] = ok ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        bool all = true;
+        float m = s_red[0];
+#pragma unroll
+        for (int w = 0; w < SW_WAVES; w++) {
+            all = all && s_ok[w];
+            m = fminf(m, s_red[w]);
+        }
+        if (all) {
+            atomicMin(&a.dtmin[slot], __float_as_uint(m));
+        } else {
+            const unsigned k = atomicAdd(&a.fb_count[slot], 1u);
+            if (k < (unsigned)a.n_deep) a.fb_list[slot * a.n_deep + k] = tile;
+        }
+    }
 }
 
 }  // namespace thy

@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("track_arrival", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("kernel", C.c_int),
                 ("strip_rows", C.c_int)]
 
-KERNELS = {"sweep": 0, "tile": 1}
+KERNELS = {"sweep": 0, "tile": 1, "sweep_general": 2}
 
 
 class Fault(C.Structure):
