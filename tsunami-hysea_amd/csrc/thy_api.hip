@@ -57,12 +57,14 @@ struct thy_ctx {
     int chunk_rows = 1;          // tile (strip) rows per XCD chunk of the step kernel's order
     int kernel = 0;              // 0 row sweep (deep-water + general split), 1 2D tiles, 2 row sweep general only
     int srows = 32;              // row sweep: rows per strip
+    int gsrows = 32;             //   rows per general strip beside the deep-water launch
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
     int order_n = 0;
     std::vector<int> order_h, cls_h;  // host copies: every block costliest first, and the classes
     int n_order = 0;             // blocks in `order` (the general launch's share)
-    int *deep = nullptr;         // deep-water blocks (n_deep of them)
-    int n_deep = 0, list_cap = 0;
+    int *deep = nullptr;         // deep-water strips (n_deep of them)
+    int *gen = nullptr;          // general strips beside them (n_gen)
+    int n_deep = 0, n_gen = 0, list_cap = 0;
     unsigned int *fb_count = nullptr;  // 3 slots
     int *fb_list = nullptr;            // [3][list_cap]
     hipGraphExec_t gexec = nullptr;
@@ -126,10 +128,14 @@ StepArgs<T> make_args(thy_ctx *c) {
         a.tiles_x = (nseg + SW_WAVES - 1) / SW_WAVES;
         a.tiles_y = (c->ny + c->srows - 1) / c->srows;
         a.srows = c->srows;
+        a.gsrows = c->gsrows;
         a.order = c->order;
         a.n_order = c->n_order;
+        a.nseg = nseg;
         a.deep = c->n_deep > 0 ? c->deep : nullptr;
         a.n_deep = c->n_deep;
+        a.gen = c->gen;
+        a.n_gen = c->n_gen;
         a.fb_count = c->fb_count;
         a.fb_list = c->fb_list;
     } else {
@@ -178,15 +184,17 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
                 hipError_t e;
                 if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(c->side, c->ev_fork, 0)) != hipSuccess) return e;
-                if (a.n_order > 0) {
-                    hipLaunchKernelGGL((k_sweep<T>), dim3(a.n_order), dim3(64 * SW_WAVES), 0, c->side, a, slot, 1);
+                const int g_deep = (a.n_deep + SW_WAVES - 1) / SW_WAVES;
+                if (a.n_gen > 0) {
+                    hipLaunchKernelGGL((k_sweep<T>), dim3((a.n_gen + SW_WAVES - 1) / SW_WAVES), dim3(64 * SW_WAVES), 0,
+                                       c->side, a, slot, 1);
                     if ((e = hipGetLastError()) != hipSuccess) return e;
                 }
-                hipLaunchKernelGGL(k_sweep_deep, dim3(a.n_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot);
+                hipLaunchKernelGGL(k_sweep_deep, dim3(g_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
                 if ((e = hipEventRecord(c->ev_join, c->side)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
-                hipLaunchKernelGGL((k_sweep<T>), dim3(a.n_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot, 2);
+                hipLaunchKernelGGL((k_sweep<T>), dim3(g_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot, 2);
                 return hipGetLastError();
             }
         }
@@ -310,35 +318,35 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
             coastal[(size_t)j * nseg + sgi] = any ? 1 : 0;
         }
     }
-    std::vector<int> cls(n_tiles, 0);
-    std::vector<unsigned char> deep_ok(n_tiles, 0);
+    // per strip (segment sgi of strip row ty): cost class, and whether the deep-water
+    // launch may take it: rows y0-4 .. y1+3 and its 64 columns clear of the grid edges
+    // (no clamping, no first-order band), of the sponges, and of coast
     const bool split = c->kernel == 0 && c->prec == 0;
     const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
-    for (int t = 0; t < n_tiles; t++) {
-        const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int n_strips = tiles_y * nseg;
+    std::vector<int> scls(n_strips, 0);
+    std::vector<unsigned char> sdeep(n_strips, 0);
+    for (int ty = 0; ty < tiles_y; ty++) {
         const int y0 = ty * S, y1 = std::min(y0 + S, ny);
         const int rows = (y1 - y0) + 8;
-        double worst = 0.0;
-        // deep-water launch: rows y0-4 .. y1+3 and each strip's 64 columns clear of the
-        // grid edges (no clamping, no first-order band), of the sponges, and of coast
-        bool dk = split && y0 - 4 >= 2 && y1 + 3 <= ny - 3 && y0 - 2 >= sps && y1 + 1 < ny - spn;
-        for (int w = 0; w < SW_WAVES; w++) {
-            const int sgi = tx * SW_WAVES + w;
-            if (sgi >= nseg) {
-                dk = false;
-                break;
-            }
+        const bool yok = y0 - 4 >= 2 && y1 + 3 <= ny - 3 && y0 - 2 >= sps && y1 + 1 < ny - spn;
+        for (int sgi = 0; sgi < nseg; sgi++) {
             const int x0 = sgi * SW_OUT;
-            if (!(x0 - SW_HALO >= 2 && x0 + SW_OUT + SW_HALO - 1 <= nx - 3 && x0 - SW_HALO >= spi &&
-                  x0 + SW_OUT + SW_HALO - 1 < nx - spd))
-                dk = false;
             int nc = 0;
             for (int j = std::max(0, y0 - 4); j < std::min(ny, y1 + 4); j++) nc += coastal[(size_t)j * nseg + sgi];
-            if (nc) dk = false;
-            worst = std::max(worst, 1.5 * nc / rows);
+            const bool xok = x0 - SW_HALO >= 2 && x0 + SW_OUT + SW_HALO - 1 <= nx - 3 && x0 - SW_HALO >= spi &&
+                             x0 + SW_OUT + SW_HALO - 1 < nx - spd;
+            scls[ty * nseg + sgi] = (int)(1.5 * nc / rows * 4.0 + 0.5);
+            sdeep[ty * nseg + sgi] = (split && yok && xok && nc == 0) ? 1 : 0;
         }
-        cls[t] = (int)(worst * 4.0 + 0.5);
-        deep_ok[t] = dk ? 1 : 0;
+    }
+    std::vector<int> cls(n_tiles, 0);
+    for (int t = 0; t < n_tiles; t++) {
+        const int ty = t / tiles_x, tx = t - ty * tiles_x;
+        for (int w = 0; w < SW_WAVES; w++) {
+            const int sgi = tx * SW_WAVES + w;
+            if (sgi < nseg) cls[t] = std::max(cls[t], scls[ty * nseg + sgi]);
+        }
     }
     // chunked XCD order as the base sequence (the kernel's order without a table)
     std::vector<int> base;
@@ -351,32 +359,64 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
         const int t = ((ch << 3) + (bid & 7)) * chunk + (loc - ch * chunk);
         if (t < n_tiles) base.push_back(t);
     }
+    // strips in the same base order: the deep-water ones as they come (neighbours stay
+    // together in a workgroup), the general ones costliest first
+    // General strips are cut into strips of G rows (items in units of G rows).
+    const int G = (S > 8 && S % 8 == 0) ? 8 : S;
+    c->gsrows = G;
+    std::vector<int> deep, gen, gcls;
+    for (int t : base) {
+        const int ty = t / tiles_x, tx = t - ty * tiles_x;
+        for (int w = 0; w < SW_WAVES; w++) {
+            const int sgi = tx * SW_WAVES + w;
+            if (sgi >= nseg) continue;
+            const int item = ty * nseg + sgi;
+            if (sdeep[item]) {
+                deep.push_back(item);
+                continue;
+            }
+            for (int sub = 0; sub < S / G && (ty * S + sub * G) < ny; sub++) {
+                gen.push_back((ty * (S / G) + sub) * nseg + sgi);
+                gcls.push_back(scls[item]);
+            }
+        }
+    }
+    {
+        std::vector<int> idx(gen.size());
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return gcls[a] > gcls[b]; });
+        std::vector<int> g2(gen.size());
+        for (size_t i = 0; i < idx.size(); i++) g2[i] = gen[idx[i]];
+        gen.swap(g2);
+    }
     std::stable_sort(base.begin(), base.end(), [&](int a, int b) { return cls[a] > cls[b]; });
     c->order_h = base;
     c->cls_h = cls;
-    std::vector<int> gen, deep;
-    for (int t : base) (deep_ok[t] ? deep : gen).push_back(t);
     if (c->gexec) {  // a captured step sequence holds the old tables and grid sizes
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
         c->gk = 0;
     }
-    if (c->list_cap < n_tiles) {
-        void *ptrs[] = {c->order, c->deep, c->fb_list};
+    const int cap = std::max(n_strips, ((ny + G - 1) / G) * nseg);
+    if (c->list_cap < cap) {
+        void *ptrs[] = {c->order, c->deep, c->gen, c->fb_list};
         for (void *p : ptrs)
             if (p) HIPCHK(hipFree(p));
-        c->order = c->deep = c->fb_list = nullptr;
-        HIPCHK(hipMalloc(&c->order, (size_t)n_tiles * sizeof(int)));
-        HIPCHK(hipMalloc(&c->deep, (size_t)n_tiles * sizeof(int)));
-        HIPCHK(hipMalloc(&c->fb_list, (size_t)3 * n_tiles * sizeof(int)));
-        c->list_cap = n_tiles;
+        c->order = c->deep = c->gen = c->fb_list = nullptr;
+        HIPCHK(hipMalloc(&c->order, (size_t)cap * sizeof(int)));
+        HIPCHK(hipMalloc(&c->deep, (size_t)cap * sizeof(int)));
+        HIPCHK(hipMalloc(&c->gen, (size_t)cap * sizeof(int)));
+        HIPCHK(hipMalloc(&c->fb_list, (size_t)3 * cap * sizeof(int)));
+        c->list_cap = cap;
     }
     if (!c->fb_count) HIPCHK(hipMalloc(&c->fb_count, 3 * sizeof(unsigned int)));
     HIPCHK(hipMemset(c->fb_count, 0, 3 * sizeof(unsigned int)));
-    c->n_order = (int)gen.size();
+    c->n_order = n_tiles;
     c->n_deep = (int)deep.size();
-    if (!gen.empty()) HIPCHK(hipMemcpy(c->order, gen.data(), gen.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->n_gen = (int)gen.size();
+    HIPCHK(hipMemcpy(c->order, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!deep.empty()) HIPCHK(hipMemcpy(c->deep, deep.data(), deep.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!gen.empty()) HIPCHK(hipMemcpy(c->gen, gen.data(), gen.size() * sizeof(int), hipMemcpyHostToDevice));
     return THY_OK;
 }
 
@@ -790,7 +830,7 @@ void thy_destroy(thy_ctx *ctx) {
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
-                    ctx->steplog, ctx->order, ctx->deep, ctx->fb_count, ctx->fb_list};
+                    ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -56,11 +56,15 @@ template <typename T> struct StepArgs {
     int nx, ny, pitch, periodic;
     int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
     int srows;                             // row-sweep kernel: rows per strip
-    const int *order;                      // row-sweep kernel: its blocks, costliest first (or null)
+    int gsrows;                            //   rows per general strip beside a deep-water launch
+    int nseg;                              // row-sweep kernel: 56-column segments per row
+    const int *order;                      // row-sweep kernel: every block, costliest first (or null)
     int n_order;                           //   number of them
-    const int *deep;                       // deep-water sweep: its blocks (null: no split)
+    const int *deep;                       // deep-water strips (strip row * nseg + segment; null: no split)
     int n_deep;
-    unsigned int *fb_count;                // 3 slots: blocks the deep-water sweep hands back
+    const int *gen;                        // general strips beside a deep-water launch, costliest first
+    int n_gen;
+    unsigned int *fb_count;                // 3 slots: strips the deep-water sweep hands back
     int *fb_list;                          // [3][n_deep]
     int sp_sup, sp_inf, sp_izq, sp_der;
     int track_arrival, n_gauges;

@@ -150,16 +150,15 @@ template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T q
     return c;
 }
 
-// One workgroup's share of a step: strip block `tile` of the grid described by `a`
-// (SW_WAVES wavefronts side by side, each a 64-column strip of a.srows rows), with
-// bc filled by begin_step for this launch.  Returns this lane's minimum of the per-cell
-// dt of the next step (3e38 when it updated no cell).
+// One wavefront's share of a step: the strip of a.srows rows whose 56 output columns
+// start at x0 and whose first row is y0, with bc filled by begin_step for this launch.
+// Returns this lane's minimum of the per-cell dt of the next step (3e38 when it
+// updated no cell, or when x0 lies past the grid).
 template <typename T>
-__device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCtl<T> &bc, const int tile,
-                                             const bool gauges) {
+__device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCtl<T> &bc, const int x0,
+                                             const int y0, const int rows) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     // (selects rather than indexing, so that a copy of the arguments stays in registers)
     const bool c1 = bc.cur != 0;
     const T *__restrict__ ge = c1 ? a.e[1] : a.e[0];
@@ -173,13 +172,7 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
     const T tnow = (T)bc.t;
     const int nx = a.nx, ny = a.ny, P = a.pitch;
     const bool per = a.periodic != 0;
-
-    if (gauges) gather_gauges(a, bc, ge, gqx, gqy, tid, 64 * SW_WAVES);
-
-    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
-    const int x0 = (txi * SW_WAVES + wv) * SW_OUT;  // first output column of this wavefront
-    const int y0 = tyi * a.srows;
-    const int y1 = min(y0 + a.srows, ny);
+    const int y1 = min(y0 + rows, ny);
     float dtmin = 3.0e38f;
 
     if (x0 < nx) {
@@ -417,6 +410,29 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
     return dtmin;
 }
 
+// The workgroup's four wavefronts on strip block `tile` (side by side in x).
+template <typename T>
+__device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCtl<T> &bc, const int tile,
+                                             const bool gauges) {
+    if (gauges) {
+        const bool c1 = bc.cur != 0;
+        gather_gauges(a, bc, c1 ? a.e[1] : a.e[0], c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], threadIdx.x,
+                      64 * SW_WAVES);
+    }
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    return sweep_strip(a, bc, (txi * SW_WAVES + wv) * SW_OUT, tyi * a.srows, a.srows);
+}
+
+// wavefront work items of the split launches: strip row * nseg + segment (-1: idle),
+// strip rows in units of `rows`
+template <typename T>
+__device__ __forceinline__ void item_origin(const StepArgs<T> &a, const int item, const int rows, int &x0, int &y0) {
+    const int strip = item / a.nseg;
+    x0 = (item - strip * a.nseg) * SW_OUT;
+    y0 = strip * rows;
+}
+
 // workgroup minimum of the lanes' dt, folded into the launch's dt slot of the grid
 template <typename T>
 __device__ __forceinline__ void block_dt_min(const StepArgs<T> &a, const int slot, float dtmin, float *s_red) {
@@ -432,11 +448,13 @@ __device__ __forceinline__ void block_dt_min(const StepArgs<T> &a, const int slo
     }
 }
 
-// One step of one grid, general faces, one workgroup per strip block.  mode 0: every
+// One step of one grid, general faces.  mode 0: one workgroup per strip block, every
 // block (the table's order, costliest first, or without a table the chunked XCD
-// order); mode 1: the table's blocks only, beside a deep-water launch; mode 2: the
-// blocks the deep-water launch of this step handed back (the grid has room for all of
-// them; surplus workgroups return at once).
+// order).  With a deep-water launch in the step, one wavefront per strip: mode 1 takes
+// the general strips (costliest first, cut into strips of a.gsrows rows so that the
+// few, slow coastal and border strips do not set the step's critical path), mode 2 the
+// strips the deep-water launch of this step handed back (the grid has room for all of
+// them; workgroups beyond the hand-backs return before the prologue).
 template <typename T>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, const int slot, const int mode) {
     __shared__ BlockCtl<T> bc;
@@ -445,23 +463,43 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
     const int bid = blockIdx.x;
     // the step's writer is the deep-water launch when there is one, else workgroup 0
     const bool writer = bid == 0 && mode == 0;
+    int n_fb = 0;
+    if (mode == 2) {  // hand-backs: workgroups beyond them return before the prologue
+        n_fb = min((int)__builtin_amdgcn_readfirstlane(a.fb_count[slot]), a.n_deep);
+        if (bid * SW_WAVES >= n_fb) return;
+    }
     if (tid == 0) begin_step(a, slot, writer, bc);
     __syncthreads();
     if (!bc.run) return;
-    int tile;
-    if (!a.order) {
-        const int loc = bid >> 3, ch = loc / a.chunk;
-        tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
-        if (tile >= a.n_tiles) return;
-    } else if (mode != 2) {
-        if (bid >= a.n_order) return;
-        tile = a.order[bid];
+    float dtmin;
+    if (mode == 0) {
+        int tile;
+        if (!a.order) {
+            const int loc = bid >> 3, ch = loc / a.chunk;
+            tile = ((ch << 3) + (bid & 7)) * a.chunk + (loc - ch * a.chunk);
+            if (tile >= a.n_tiles) return;
+        } else {
+            if (bid >= a.n_order) return;
+            tile = a.order[bid];
+        }
+        dtmin = sweep_block(a, bc, __builtin_amdgcn_readfirstlane(tile), writer);
     } else {
-        const int n_fb = min((int)__builtin_amdgcn_readfirstlane(a.fb_count[slot]), a.n_deep);
-        if (bid >= n_fb) return;
-        tile = a.fb_list[slot * a.n_deep + bid];
+        const int k = bid * SW_WAVES + __builtin_amdgcn_readfirstlane(tid >> 6);
+        int item = -1;
+        if (mode == 1) {
+            if (k < a.n_gen) item = a.gen[k];
+        } else {
+            if (k < n_fb) item = a.fb_list[slot * a.n_deep + k];
+        }
+        item = __builtin_amdgcn_readfirstlane(item);
+        const int rows = mode == 1 ? a.gsrows : a.srows;
+        dtmin = 3.0e38f;
+        if (item >= 0) {
+            int x0, y0;
+            item_origin(a, item, rows, x0, y0);
+            dtmin = sweep_strip(a, bc, x0, y0, rows);
+        }
     }
-    const float dtmin = sweep_block(a, bc, __builtin_amdgcn_readfirstlane(tile), writer);
     block_dt_min(a, slot, dtmin, s_red);
 }
 
@@ -477,74 +515,86 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
 // launches evaluate the deep faces with the same functions, so the split changes no
 // result.
 // ---------------------------------------------------------------------------
+// buffer resource over a whole device array (raw, 32-bit offsets)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t whole_array(const void *p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+
+// One wavefront's strip on the deep-water path.  The 2s faces reach one row and one
+// column, so the pipeline lags two rows: iteration r loads row r, evaluates the mass
+// faces of row r and the y face between rows r-1 and r, h* of row r-1, the momentum
+// faces of row r-1 and the y face between rows r-2 and r-1, and updates row r-2.  Per
+// cell the accumulation order is that of the general pipeline (x faces, lower y face,
+// upper y face), so the values are the same.  The conditions under which the general
+// faces take the 2s branch are checked for the whole strip and combined: every row
+// y0-4 .. y1+3 and every h* row y0-2 .. y1+1 deeper than hpos on every lane.  Returns
+// whether they held; if not, the caller hands the strip back (its writes are then
+// rewritten by the general launch and its dt is dropped).
 template <typename T>
-__device__ __forceinline__ bool deep_block(const StepArgs<T> &a, const BlockCtl<T> &bc, const int tile,
+__device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<T> &bc, const int x0, const int y0,
                                            float &dtmin_out) {
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    static_assert(sizeof(T) == 4, "the deep-water path is FP32");
+    const int lane = threadIdx.x & 63;
     const bool c1 = bc.cur != 0;
-    const T *__restrict__ ge = c1 ? a.e[1] : a.e[0];
-    const T *__restrict__ gqx = c1 ? a.qx[1] : a.qx[0];
-    const T *__restrict__ gqy = c1 ? a.qy[1] : a.qy[0];
-    const T *__restrict__ gD = a.D;
     T *__restrict__ oe = c1 ? a.e[0] : a.e[1];
     T *__restrict__ oqx = c1 ? a.qx[0] : a.qx[1];
     T *__restrict__ oqy = c1 ? a.qy[0] : a.qy[1];
     const T dt = (T)bc.dt;
     const T tnow = (T)bc.t;
-    const int nx = a.nx, P = a.pitch;
-    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
-    const int x0 = (txi * SW_WAVES + wv) * SW_OUT;
-    const int y0 = tyi * a.srows;
+    const int P = a.pitch;
     const int y1 = min(y0 + a.srows, a.ny);
     float dtmin = 3.0e38f;
-    bool ok = true;
-    if (x0 < nx) {
-        // the block classification keeps columns x0-4 .. x0+59 and rows y0-4 .. y1+3
-        // inside the grid: no clamping
-        const unsigned gx = (unsigned)(x0 - SW_HALO + lane);
-        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT;
-        const T hpos = a.hpos, cd2s = T(0.5) * (T(1) - a.cvis);
-        const T *__restrict__ gtar = a.track_arrival ? a.tarr : a.emax;
-        const T *__restrict__ gini = a.track_arrival ? a.eini : a.emax;
-        auto load_row = [&](int r) {
-            const int64_t o = (int64_t)r * P;
+    bool valid = true;
+    if (x0 < a.nx) {
+        // the strip classification keeps columns x0-4 .. x0+59 and rows y0-4 .. y1+3
+        // inside the grid: no clamping.  Loads are buffer loads: the lane's column is the
+        // vector offset, the row the scalar offset.
+        const int bytes = P * a.ny * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t re = whole_array(c1 ? a.e[1] : a.e[0], bytes);
+        const __amdgpu_buffer_rsrc_t rqx = whole_array(c1 ? a.qx[1] : a.qx[0], bytes);
+        const __amdgpu_buffer_rsrc_t rqy = whole_array(c1 ? a.qy[1] : a.qy[0], bytes);
+        const __amdgpu_buffer_rsrc_t rD = whole_array(a.D, bytes);
+        const __amdgpu_buffer_rsrc_t rmax = whole_array(a.emax, bytes);
+        const __amdgpu_buffer_rsrc_t rtar = whole_array(a.track_arrival ? a.tarr : a.emax, bytes);
+        const __amdgpu_buffer_rsrc_t rini = whole_array(a.track_arrival ? a.eini : a.emax, bytes);
+        const int gx = x0 - SW_HALO + lane;
+        const int voff = gx * (int)sizeof(T);
+        auto ld = [&](const __amdgpu_buffer_rsrc_t r, const int row) {
+            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, row * P * (int)sizeof(T), 0));
+        };
+        auto load_row = [&](const int r) {
             RowCell<T> c;
-            c.e = (ge + o)[gx];
-            c.D = (gD + o)[gx];
-            c.qx = (gqx + o)[gx];
-            c.qy = (gqy + o)[gx];
+            c.e = ld(re, r);
+            c.D = ld(rD, r);
+            c.qx = ld(rqx, r);
+            c.qy = ld(rqy, r);
             return c;
         };
-        auto load_metrics = [&](int jn, T &m, T &t, T &i) {
-            const int64_t o = (int64_t)(jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P;
-            m = (a.emax + o)[gx];
-            t = (gtar + o)[gx];
-            i = (gini + o)[gx];
-        };
-        // pipeline registers: input rows r-4..r, e* rows r-4..r-3 with sqrt(h*) and qy/h*,
-        // mass accumulators rows r-2..r-1, momentum accumulators rows r-4..r-3
-        RowCell<T> i4, i3, i2, i1, i0;
-        i4 = i3 = i2 = i1 = i0 = RowCell<T>{T(0), T(0), T(0), T(0)};
-        T s4 = T(0), s3 = T(0), sq4 = T(0), sq3 = T(0), v4 = T(0), v3 = T(0);
-        T m2 = T(0), m1 = T(0);
-        T X4 = T(0), Y4 = T(0), W4 = T(0), X3 = T(0), Y3 = T(0), W3 = T(0);
-        const int rbeg = y0 - 4, rend = y1 + 3;
-        RowCell<T> buf = load_row(rbeg);
-        T pm, pt, pi;
-        load_metrics(rbeg - 4, pm, pt, pi);
-        for (int r = rbeg; r <= rend; r++) {
-            i0 = buf;
-            buf = load_row(r < rend ? r + 1 : r);
-            const int ju = r - 4;
+        auto metrics_row = [&](const int j) { return j < y0 ? y0 : (j >= y1 ? y1 - 1 : j); };
+        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT;
+        const T hpos = a.hpos, cd2s = T(0.5) * (T(1) - a.cvis);
+        // rows y0-4 and y1+3 enter only the branch conditions of the outermost mass faces
+        valid = __all(ld(re, y0 - 4) + ld(rD, y0 - 4) > hpos) && __all(ld(re, y1 + 3) + ld(rD, y1 + 3) > hpos);
+        // pipeline registers: input rows r-2, r-1; h* row r-2 with sqrt(h*), qy/h*;
+        // mass accumulator of row r-1; momentum accumulators of row r-2
+        RowCell<T> i2 = RowCell<T>{T(0), T(0), T(0), T(0)}, i1 = i2;
+        T s2 = T(0), sq2 = T(0), v2 = T(0), m1 = T(0);
+        T X2 = T(0), Y2 = T(0), W2 = T(0);
+        RowCell<T> buf = load_row(y0 - 3);
+        T pm = ld(rmax, y0), pt = ld(rtar, y0), pi = ld(rini, y0);
+        for (int r = y0 - 3; r <= y1 + 2; r++) {
+            const RowCell<T> i0 = buf;
+            buf = load_row(r < y1 + 2 ? r + 1 : r);
+            const int ju = r - 2;  // row updated in this iteration
             const T cmax = pm, ctar = pt, cini = pi;
-            load_metrics(ju + 1, pm, pt, pi);
-            const T h0 = i0.e + i0.D;
-            if (!__all(h0 > hpos)) {
-                ok = false;
-                break;
+            {
+                const int jn = metrics_row(ju + 1);
+                pm = ld(rmax, jn);
+                pt = ld(rtar, jn);
+                pi = ld(rini, jn);
             }
+            const T h0 = i0.e + i0.D;
+            valid = valid && __all(h0 > hpos);
             // mass x-faces of row r (h* rows y0-2 .. y1+1)
             T m0 = T(0);
             if (r >= y0 - 2 && r <= y1 + 1) {
@@ -552,87 +602,81 @@ __device__ __forceinline__ bool deep_block(const StepArgs<T> &a, const BlockCtl<
                 const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
                 m0 = F - from_right(F);
             }
-            // mass y-face g = r-1 (faces y0-2 .. y1+2)
-            if (r - 1 >= y0 - 2 && r - 1 <= y1 + 2) {
-                const FaceGeoY<T> fg = uniform_load(a.fy + (r - 1));
-                const T F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
-                m2 -= F * fg.ic0;
-                m1 += F * fg.ic1;
+            // mass y-face g = r between rows r-1 and r (faces y0-2 .. y1+2)
+            if (r >= y0 - 2) {
+                const FaceGeoY<T> fg = uniform_load(a.fy + r);
+                const T F = fast_mass_y(i1.qy, i1.e + i1.D, h0, fg, dt);
+                m1 -= F * fg.ic0;
+                m0 += F * fg.ic1;
             }
-            // h* of row r-2
-            T s2 = T(0), hs2 = T(0), sq2 = T(0), u2 = T(0), v2 = T(0);
-            const bool hrow = r - 2 >= y0 - 2 && r - 2 <= y1 + 1;
-            const RowGeo<T> rg2 = uniform_load(a.row + (r - 2 < y0 - 4 ? y0 - 4 : r - 2));
-            if (hrow) {
-                T es = i2.e + dt * rg2.invA * m2;
-                if (!(es + i2.D > T(0))) es = -i2.D;
-                s2 = es;
-                hs2 = es + i2.D;
-                if (!__all(hs2 > hpos)) {
-                    ok = false;
-                    break;
-                }
-                sq2 = Num<T>::sqrt(hs2);
-                const T ih = Num<T>::rcp(hs2);
-                u2 = i2.qx * ih;
-                v2 = i2.qy * ih;
+            // h* of row r-1 (rows y0-2 .. y1+1)
+            T s1 = T(0), hs1 = T(0), sq1 = T(0), u1 = T(0), v1 = T(0);
+            const int j1 = r - 1;
+            const RowGeo<T> rg1 = uniform_load(a.row + (j1 < y0 - 2 ? y0 - 2 : j1));
+            if (j1 >= y0 - 2 && j1 <= y1 + 1) {
+                T es = i1.e + dt * rg1.invA * m1;
+                if (!(es + i1.D > T(0))) es = -i1.D;
+                s1 = es;
+                hs1 = es + i1.D;
+                valid = valid && __all(hs1 > hpos);
+                sq1 = Num<T>::sqrt(hs1);
+                const T ih = Num<T>::rcp(hs1);
+                u1 = i1.qx * ih;
+                v1 = i1.qy * ih;
             }
-            // momentum x-faces of row r-2 (rows y0 .. y1-1)
-            T X2 = T(0), Y2 = T(0), W2 = T(0);
-            if (r - 2 >= y0 && r - 2 < y1) {
+            // momentum x-faces of row r-1 (rows y0 .. y1-1)
+            T X1 = T(0), Y1 = T(0), W1 = T(0);
+            if (j1 >= y0 && j1 < y1) {
                 DC<T> A, B;
-                A.h = from_left(hs2); A.e = from_left(s2); A.D = from_left(i2.D);
-                A.qn = from_left(i2.qx); A.qt = from_left(i2.qy); A.s = from_left(sq2); A.un = from_left(u2);
-                B.h = hs2; B.e = s2; B.D = i2.D; B.qn = i2.qx; B.qt = i2.qy; B.s = sq2; B.un = u2;
-                const MF<T> f = fast_mom_x(A, B, rg2.lenx, rg2.gx, rg2.valx, dt, cd2s);
-                X2 = -f.pn - from_right(f.mn);
-                Y2 = -f.pt - from_right(f.mt);
-                W2 = f.w + from_right(f.w);
+                A.h = from_left(hs1); A.e = from_left(s1); A.D = from_left(i1.D);
+                A.qn = from_left(i1.qx); A.qt = from_left(i1.qy); A.s = from_left(sq1); A.un = from_left(u1);
+                B.h = hs1; B.e = s1; B.D = i1.D; B.qn = i1.qx; B.qt = i1.qy; B.s = sq1; B.un = u1;
+                const MF<T> f = fast_mom_x(A, B, rg1.lenx, rg1.gx, rg1.valx, dt, cd2s);
+                X1 = -f.pn - from_right(f.mn);
+                Y1 = -f.pt - from_right(f.mt);
+                W1 = f.w + from_right(f.w);
             }
-            // momentum y-face g = r-3 (faces y0 .. y1)
-            if (r - 3 >= y0 && r - 3 <= y1) {
-                const FaceGeoY<T> fg = uniform_load(a.fy + (r - 3));
+            // momentum y-face g = r-1 between rows r-2 and r-1 (faces y0 .. y1)
+            if (j1 >= y0 && j1 <= y1) {
+                const FaceGeoY<T> fg = uniform_load(a.fy + j1);
                 DC<T> A, B;
-                A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
-                B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
+                A.h = s2 + i2.D; A.e = s2; A.D = i2.D; A.qn = i2.qy; A.qt = i2.qx; A.s = sq2; A.un = v2;
+                B.h = hs1; B.e = s1; B.D = i1.D; B.qn = i1.qy; B.qt = i1.qx; B.s = sq1; B.un = v1;
                 const MF<T> f = fast_mom_y(A, B, fg, dt, cd2s);
-                Y4 -= f.mn;
-                X4 -= f.mt;
-                W4 += f.w;
-                Y3 -= f.pn;
-                X3 -= f.pt;
-                W3 += f.w;
+                Y2 -= f.mn;
+                X2 -= f.mt;
+                W2 += f.w;
+                Y1 -= f.pn;
+                X1 -= f.pt;
+                W1 += f.w;
             }
-            // cell update of row r-4 (src/GPU/Volumen_kernel.cu:160-221; h* > hpos here)
+            // cell update of row r-2 (src/GPU/Volumen_kernel.cu:160-221; h* > hpos here)
             if (ju >= y0 && ju < y1 && out_col) {
                 const RowGeo<T> rg = uniform_load(a.row + ju);
                 const int64_t go = (int64_t)ju * P + gx;
-                const T D = i4.D, eo = i4.e;
+                const T D = i2.D, eo = i2.e;
                 if (eo > cmax && eo + D > a.eps_h) a.emax[go] = eo;
                 if (a.track_arrival && ctar < T(0) && tabs(eo - cini) > a.dif_at) a.tarr[go] = tnow;
-                const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
+                const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
                 dtmin = fminf(dtmin, (float)dtc);
                 const T val = dt * rg.invA;
-                oe[go] = s4;
-                oqx[go] = i4.qx + val * X4;
-                oqy[go] = i4.qy + val * Y4;
+                oe[go] = s2;
+                oqx[go] = i2.qx + val * X2;
+                oqy[go] = i2.qy + val * Y2;
             }
-            i4 = i3; i3 = i2; i2 = i1; i1 = i0;
-            s4 = s3; s3 = s2;
-            sq4 = sq3; sq3 = sq2; v4 = v3; v3 = v2;
-            m2 = m1; m1 = m0;
-            X4 = X3; Y4 = Y3; W4 = W3;
-            X3 = X2; Y3 = Y2; W3 = W2;
+            i2 = i1; i1 = i0;
+            s2 = s1; sq2 = sq1; v2 = v1;
+            m1 = m0;
+            X2 = X1; Y2 = Y1; W2 = W1;
         }
     }
     dtmin_out = dtmin;
-    return ok;
+    return valid;
 }
 
 __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<float> a, const int slot) {
     __shared__ BlockCtl<float> bc;
     __shared__ float s_red[SW_WAVES];
-    __shared__ int s_ok[SW_WAVES];
     const int tid = threadIdx.x, bid = blockIdx.x;
     if (tid == 0) {
         begin_step(a, slot, bid == 0, bc);  // this launch is the step's writer
@@ -645,31 +689,22 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<flo
         gather_gauges(a, bc, c1 ? a.e[1] : a.e[0], c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], tid,
                       64 * SW_WAVES);
     }
-    if (bid >= a.n_deep) return;
-    const int tile = a.deep[bid];
-    float dtmin;
-    const bool ok = deep_block(a, bc, tile, dtmin);
-    dtmin = wave_min(dtmin);
-    if ((tid & 63) == 0) {
-        s_red[tid >> 6] = dtmin;
-        s_ok[tid >> 6] = ok ? 1 : 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        bool all = true;
-        float m = s_red[0];
-#pragma unroll
-        for (int w = 0; w < SW_WAVES; w++) {
-            all = all && s_ok[w];
-            m = fminf(m, s_red[w]);
-        }
-        if (all) {
-            atomicMin(&a.dtmin[slot], __float_as_uint(m));
-        } else {
-            const unsigned k = atomicAdd(&a.fb_count[slot], 1u);
-            if (k < (unsigned)a.n_deep) a.fb_list[slot * a.n_deep + k] = tile;
+    const int k = bid * SW_WAVES + __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int item = __builtin_amdgcn_readfirstlane(k < a.n_deep ? a.deep[k] : -1);
+    float dtmin = 3.0e38f;
+    if (item >= 0) {
+        int x0, y0;
+        item_origin(a, item, a.srows, x0, y0);
+        if (!deep_strip(a, bc, x0, y0, dtmin)) {
+            // hand the strip back; its dt comes from the general launch that redoes it
+            dtmin = 3.0e38f;
+            if ((tid & 63) == 0) {
+                const unsigned n = atomicAdd(&a.fb_count[slot], 1u);
+                if (n < (unsigned)a.n_deep) a.fb_list[slot * a.n_deep + n] = item;
+            }
         }
     }
+    block_dt_min(a, slot, dtmin, s_red);
 }
 
 }  // namespace thy
