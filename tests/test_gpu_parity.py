@@ -317,6 +317,17 @@ def test_deep_split_identical_to_general_faces():
         assert np.array_equal(x, y)
 
 
+def test_deep_split_identical_on_serving_grid():
+    """The 1000x800 shared serving grid (open ocean, shelf and coast in one strip
+    row): 60 steps of the split run equal the general-faces-only run bit for bit, so
+    no formula is rounded differently in the two kernels."""
+    from thysea import scenario as S
+    pr = S.batch_problems(1)[0]
+    a, b = _split_pair(pr, 60)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y), float(np.max(np.abs(x - y)))
+
+
 def test_deep_split_hands_back_failing_blocks():
     """A dry patch carved into deep water (classified deep from the still-water depth,
     but h <= hpos at run time): the deep-water launch hands those blocks back to the
@@ -345,3 +356,28 @@ def test_deep_split_hands_back_failing_blocks():
     wet = ho > pr.eps_h
     assert rel_linf(hg[wet], ho[wet], np.max(np.abs((ho - Ho)[wet]))) < 1e-3
     s.close()
+
+
+def test_batch_matches_alone_with_deep_water_strips():
+    """A grid large enough for the deep-water launch: the batch (general faces on every
+    strip) gives the same max height, arrival times and gauges as the scenario run
+    alone (deep-water launch beside the general one)."""
+    nat = _native()
+    pr = _scenario(400, 300, 13, 0.4, 3.0, 900.0)
+    s = nat.Solver(pr)
+    ra = s.run(steps_per_sync=6)
+    ma, aa = s.field("max_height"), s.field("arrival_times")
+    s.close()
+    b = nat.Batch(capacity=2, steps_per_sync=6)
+    sb = nat.Solver(pr)
+    slot = b.add(sb)
+    while b.advance() > 0:
+        pass
+    rb = b.result(slot)
+    assert rb["steps"] == ra["steps"]
+    d = np.abs(rb["max_height"] - ma)
+    assert np.array_equal(rb["max_height"], ma), float(d.max())
+    assert np.array_equal(rb["arrival_times"], aa)
+    assert np.array_equal(rb["g_eta"], ra["g_eta"])
+    b.close()
+    sb.close()

@@ -94,11 +94,18 @@ template <typename T> __device__ __forceinline__ T blend(T hij, T hpos, bool alf
     return T(1);
 }
 
-// positivity limiter: src/GPU/Arista_kernel.cu:79-114 (g0/g1 = 1 for x faces, cos for y)
+// positivity limiter: src/GPU/Arista_kernel.cu:79-114 (g0/g1 = 1 for x faces, cos for y).
+// The reference limits the flux when dt exceeds dt0 = 0.25 h g A / (|F| + eps); the
+// test is evaluated as dt (|F| + eps) > 0.25 h g A (the same condition, no division
+// unless the limiter acts).  Every face evaluation (deep-water forms and general faces)
+// goes through this one function, so a face evaluated by two wavefronts has one value.
 template <typename T> __device__ __forceinline__ T positivity(T F, T h0, T h1, T g0, T g1, T dt, T A0, T A1) {
-    T dt0 = (F > T(0)) ? Num<T>::div(T(0.25) * h0 * g0 * A0, F + Num<T>::eps)
-                       : Num<T>::div(T(0.25) * h1 * g1 * A1, -F + Num<T>::eps);
-    if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
+    const T lim = (F > T(0)) ? T(0.25) * h0 * g0 * A0 : T(0.25) * h1 * g1 * A1;
+    const T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    if (dt * den > lim) {
+        const T dt0 = Num<T>::div(lim, den);
+        if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
+    }
     return F;
 }
 
@@ -194,11 +201,12 @@ template <typename T> struct MF {
 // Momentum (step 2) face flux.
 //   x: src/GPU/Arista_kernel.cu:221-464,565-670   y: :780-1026,1127-1233
 // For x faces: g0 = g1 = 1/cos(phi_row) (len*g = len/cos), qn = qx, qt = qy.
-// For y faces: g0 = cos0, g1 = cos1, qn = qy, qt = qx.
+// For y faces: g0 = cos0, g1 = cos1 with ic0 = 1/cos0, ic1 = 1/cos1 (the row table's
+// values, shared with the deep-water forms), qn = qy, qt = qx.
 // ---------------------------------------------------------------------------
 template <typename T, bool YFACE>
 __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int border, bool alfa_cero,
-                                          T len, T g0, T g1, T valf, T dt, const Prm<T> &p) {
+                                          T len, T g0, T g1, T ic0, T ic1, T valf, T dt, const Prm<T> &p) {
     MF<T> o;
     if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
         // deep-water 2s branch
@@ -215,7 +223,6 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         T qnn = up ? b.qn : a.qn;
         T qtt = up ? b.qt : a.qt;
         if (YFACE) {
-            T ic0 = Num<T>::rcp(g0), ic1 = Num<T>::rcp(g1);
             deta = T(0.5) * (g0 + g1) * deta - uij * uij * (g1 - g0);
             T fn = uu * qnn * len;
             T mn, pn;
@@ -243,12 +250,15 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         return o;
     }
     T normal = orient(border, p, m, a, b, q);
-    if (border == 1) { T t = g0; g0 = g1; g1 = t; }
+    if (border == 1) {
+        T t = g0; g0 = g1; g1 = t;
+        t = ic0; ic0 = ic1; ic1 = t;
+    }
     if (!(a.h > Num<T>::eps || b.h > Num<T>::eps)) {
         o.mn = o.mt = o.pn = o.pt = o.w = T(0);
         return o;
     }
-    T ic0 = YFACE ? Num<T>::rcp(g0) : T(1), ic1 = YFACE ? Num<T>::rcp(g1) : T(1);
+    if (!YFACE) ic0 = ic1 = T(1);
     T L = YFACE ? len : len * g0;   // face length factor (len/cos for x)
     T h0 = a.h, h1 = b.h;
     T q0 = a.qn * normal, q1 = b.qn * normal;

@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             const int sj = tj + 2;
             SC<T> m = st_cell(cxs(fi), sj), c0 = st_cell(cxs(fi + 1), sj);
             SC<T> c1 = st_cell(cxs(fi + 2), sj), q = st_cell(cxs(fi + 3), sj);
-            const MF<T> f = mom_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.valx, dt, px);
+            const MF<T> f = mom_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, T(1), T(1), rg.valx, dt, px);
             const bool lo = fi >= 1;
             const bool hi = fi <= TX - 1 && (per || g <= nx - 1);
             if (lo) {
@@ -733,7 +733,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             t0 = c0.qn; c0.qn = c0.qt; c0.qt = t0;
             t0 = c1.qn; c1.qn = c1.qt; c1.qt = t0;
             t0 = q.qn; q.qn = q.qt; q.qt = t0;
-            const MF<T> f = mom_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.valy, dt, py);
+            const MF<T> f = mom_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1,
+                                              fg.valy, dt, py);
             const bool lo = fj >= 1;
             const bool hi = fj <= TY - 1 && g <= ny - 1;
             if (lo) {

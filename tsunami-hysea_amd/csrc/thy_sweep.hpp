@@ -48,32 +48,21 @@ template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
 // Deep-water forms of the face fluxes: the 2s branch that mass_flux / mom_flux take
 // when all four stencil depths exceed hpos outside the first-order band
 // (src/GPU/Arista_kernel.cu:545,653,1105,1214), written with the per-cell square
-// roots and velocities computed once per cell, and the positivity limiter evaluated
-// without a division unless it acts.
+// roots and velocities computed once per cell (the same arithmetic as the per-lane 2s
+// branch of the general faces, so either path gives a face the same value).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T fast_positivity(T F, T lim0, T lim1, T dt) {
-    const T lim = (F > T(0)) ? lim0 : lim1;
-    const T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
-    if (dt * den > lim) {
-        const T dt0 = Num<T>::div(lim, den);
-        if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
-    }
-    return F;
-}
-
 // x face between cells a (left) and b: src/GPU/Arista_kernel.cu:79-95,118-130
 template <typename T>
 __device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T len, T g, T A, T dt) {
     const T F = qa * len * g;
-    return fast_positivity(F, T(0.25) * ha * T(1) * A, T(0.25) * hb * T(1) * A, dt);
+    return positivity(F, ha, hb, T(1), T(1), dt, A, A);
 }
 
 // y face between rows a (g-1) and b (g): src/GPU/Arista_kernel.cu:97-114,674-686
 template <typename T>
 __device__ __forceinline__ T fast_mass_y(T qa, T ha, T hb, const FaceGeoY<T> &f, T dt) {
     const T F = f.len * qa * f.cos0;
-    return fast_positivity(F, T(0.25) * ha * f.cos0 * f.A0, T(0.25) * hb * f.cos1 * f.A1, dt);
+    return positivity(F, ha, hb, f.cos0, f.cos1, dt, f.A0, f.A1);
 }
 
 // one cell of a deep momentum face: total depth, eta, depth, normal and tangential
@@ -335,7 +324,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 const T eR = from_right(s2), DR = from_right(i2.D), qyR = from_right(i2.qy);
                 const SC<T> M = sc_of(eLL, DLL, T(0), qyLL), A = sc_of(eL, DL, qxL, qyL);
                 const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy), Q = sc_of(eR, DR, T(0), qyR);
-                const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.lenx, rg2.gx, rg2.gx, rg2.valx, dt, px);
+                const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.lenx, rg2.gx, rg2.gx, T(1), T(1),
+                                                   rg2.valx, dt, px);
                 X2 = -((xb == 1) ? f.mn : f.pn) - from_right(f.mn);
                 Y2 = -((xb == 1) ? f.mt : f.pt) - from_right(f.mt);
                 W2 = f.w + from_right(f.w);
@@ -360,7 +350,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     if (g - 1 < 0) A = B;
                     if (g >= ny) B = A;
                     if (g + 1 >= ny) Q = (g >= ny) ? A : B;
-                    f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.valy, dt, py);
+                    f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1, fg.valy, dt,
+                                          py);
                 }
                 Y4 -= f.mn;
                 X4 -= f.mt;
