@@ -190,14 +190,20 @@ int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);
 
 /* ---- Batched scenarios --------------------------------------------------------
    No counterpart in the reference, which runs one scenario per process
-   (src/GPU/ShallowWater.cu:182-190).  A batch advances many contexts together: each
-   step launch is one persistent kernel that takes (scenario, strip block) work items
-   from a device queue, and every scenario keeps its own dt, time, faults, gauges and
+   (src/GPU/ShallowWater.cu:182-190).  A batch advances many contexts together: a step
+   of the batch is three launches over the strips of every running scenario (the
+   deep-water strips, the general strips beside them, then the strips the deep-water
+   launch handed back), and every scenario keeps its own dt, time, faults, gauges and
    metrics, so its results are those of thy_run on the same context.  Contexts are
    created and uploaded by the caller (thy_create, thy_upload_state), admitted with
    their run configuration, and stay owned by the caller; results are read with
    thy_get_field / thy_get_gauges once the slot is THY_SLOT_FINISHED.  FP32 row-sweep
-   contexts only; no snapshots (save_interval < 0). */
+   contexts only; no snapshots (save_interval < 0).
+
+   The step sequence can run asynchronously: thy_batch_launch returns as soon as it is
+   queued, thy_batch_wait collects it.  Between the two the caller may admit
+   scenarios (they join the next launch), read and remove finished ones, and create
+   or destroy other contexts, while the GPU steps the batch. */
 typedef struct thy_batch thy_batch;
 
 enum thy_slot_state { THY_SLOT_FREE = 0, THY_SLOT_RUNNING = 1, THY_SLOT_FINISHED = 2 };
@@ -210,14 +216,27 @@ const char *thy_batch_last_error(thy_batch *b);
 /* Prepares the run (metrics, initial dt, faults due at t = 0) and admits the context;
    THY_ERR_BUDGET when no slot is free or its footprint would exceed the budget. */
 int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot);
-/* One captured sequence of steps_per_sync launches for every running scenario, then
-   the host work between them (faults that came due, finished scenarios). */
+/* Queues one captured sequence of steps_per_sync steps for every running scenario and
+   returns; THY_ERR_INVALID while a launch is in flight.  Does nothing when no
+   scenario is running. */
+int thy_batch_launch(thy_batch *b);
+/* *done = 1 when no launch is in flight or it has finished (never waits). */
+int thy_batch_ready(thy_batch *b, int *done);
+/* Waits for the launch in flight (if any) and does the host work after it: faults
+   that came due, finished scenarios.  *n_running: scenarios running afterwards. */
+int thy_batch_wait(thy_batch *b, int *n_running);
+/* thy_batch_launch, then thy_batch_wait. */
 int thy_batch_advance(thy_batch *b, int *n_running);
 int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res);
 /* Frees a slot (the context is not destroyed; remove a running scenario's slot before
-   destroying its context). */
+   destroying its context).  Removing a running slot waits for a launch in flight. */
 int thy_batch_remove(thy_batch *b, int slot);
 int thy_batch_info(thy_batch *b, int *running, size_t *bytes_used, double *gpu_seconds);
+
+/* Releases the device memory, pinned host memory and streams that destroyed contexts
+   and batches left in the library's caches (they are otherwise kept for reuse until
+   the process ends).  Call with no context or batch alive. */
+void thy_release_cached(void);
 
 #ifdef __cplusplus
 }

@@ -166,8 +166,8 @@ def _scenario(nx, ny, seed, fault_frac, slip, t_sim):
 def test_batch_matches_alone():
     """Scenarios of different sizes and durations, advanced together in one batch,
     give the results of each run alone (same kernel code, own dt per scenario): max
-    height, arrival times, gauges and step counts identical; one scenario admitted
-    while the others are in flight."""
+    height, arrival times, gauges and step counts identical; one scenario created,
+    uploaded and admitted while a launch of the others is in flight."""
     nat = _native()
     probs = [_scenario(96, 80, 3, 0.4, 2.0, 600.0), _scenario(150, 64, 5, 0.5, 4.0, 900.0),
              _scenario(70, 120, 9, 0.3, 1.0, 400.0)]
@@ -180,10 +180,12 @@ def test_batch_matches_alone():
         alone.append(r)
         s.close()
     b = nat.Batch(capacity=4, steps_per_sync=6)
-    solvers = [nat.Solver(pr) for pr in probs]
+    solvers = [nat.Solver(pr) for pr in probs[:2]]
     slots = [b.add(solvers[0]), b.add(solvers[1])]
-    b.advance()
-    slots.append(b.add(solvers[2]))  # admitted mid-flight
+    b.launch()
+    solvers.append(nat.Solver(probs[2]))
+    slots.append(b.add(solvers[2]))  # admitted while the launch is in flight
+    assert b.wait() == 3
     for _ in range(10000):
         if b.advance() == 0:
             break
@@ -200,6 +202,25 @@ def test_batch_matches_alone():
     b.close()
     for s in solvers:
         s.close()
+
+
+def test_serving_loop_matches_sequential():
+    """Requests arriving over time, served by the pipelined loop (admission and
+    read-out while the GPU steps the batch, finished slots removed while a launch is
+    in flight), give the results of the same requests run one at a time."""
+    from thysea import serving
+    probs = [_scenario(96, 80, 3, 0.4, 2.0, 300.0), _scenario(150, 64, 5, 0.5, 4.0, 500.0),
+             _scenario(70, 120, 9, 0.3, 1.0, 200.0), _scenario(400, 300, 13, 0.4, 3.0, 400.0),
+             _scenario(96, 80, 4, 0.6, 2.5, 250.0)]
+    arrivals = [0.0, 0.0, 0.02, 0.05, 0.08]
+    keep = list(range(len(probs)))
+    rec_b, res_b, _ = serving.run_batched(probs, arrivals=arrivals, steps_per_sync=6, keep=keep, burst=2)
+    rec_s, res_s = serving.run_sequential(probs, arrivals=arrivals, steps_per_sync=6, keep=keep)
+    for i in keep:
+        assert rec_b[i]["steps"] == rec_s[i]["steps"] > 0
+        for key in ("max_height", "arrival_times", "g_eta"):
+            assert np.array_equal(res_b[i][key], res_s[i][key]), (i, key)
+        assert rec_b[i]["finished"] >= rec_b[i]["admitted"] >= rec_b[i]["arrival"]
 
 
 def test_batch_budget_admission():
@@ -333,12 +354,7 @@ def test_deep_split_hands_back_failing_blocks():
     but h <= hpos at run time): the deep-water launch hands those blocks back to the
     general launch of the same step; the result is still bit-identical to the
     general-faces-only run and within tolerance of the oracle."""
-    pr = make_problem(nx=400, ny=300, seamounts=0, seed=22, fault=False)
-    H = pr.H.reshape(pr.ny, pr.nx)
-    h = pr.h.reshape(pr.ny, pr.nx).copy()
-    assert (H[140:150, 180:190] > 1.0).all()  # deep water (nondimensional depth > 1000 m)
-    h[140:150, 180:190] = 0.0                 # dry hole: the ocean rushes in (WAF, wet/dry)
-    pr.h = h.ravel() if pr.h.ndim == 1 else h
+    pr = _hole_problem()
     a, b = _split_pair(pr, 20)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
@@ -356,6 +372,44 @@ def test_deep_split_hands_back_failing_blocks():
     wet = ho > pr.eps_h
     assert rel_linf(hg[wet], ho[wet], np.max(np.abs((ho - Ho)[wet]))) < 1e-3
     s.close()
+
+
+def _hole_problem():
+    """Flat deep basin with a dry hole carved into it (deep-water strips by the still
+    water depth, h <= hpos at run time)."""
+    pr = make_problem(nx=400, ny=300, seamounts=0, seed=22, fault=False)
+    H = pr.H.reshape(pr.ny, pr.nx)
+    h = pr.h.reshape(pr.ny, pr.nx).copy()
+    assert (H[140:150, 180:190] > 1.0).all()
+    h[140:150, 180:190] = 0.0
+    pr.h = h.ravel() if pr.h.ndim == 1 else h
+    return pr
+
+
+def test_batch_hands_back_failing_deep_strips():
+    """The dry hole next to a scenario with a fault, in one batch: the batch hands the
+    hole's strips back to its hand-back launch, and both scenarios end in the state of
+    their runs alone, bit for bit."""
+    nat = _native()
+    probs = [_hole_problem(), _scenario(400, 300, 13, 0.4, 3.0, 900.0)]
+    alone = []
+    for pr in probs:
+        s = nat.Solver(pr)
+        s.run(max_steps=30, steps_per_sync=6)
+        alone.append(_fields(s))
+        s.close()
+    b = nat.Batch(capacity=2, steps_per_sync=6)
+    solvers = [nat.Solver(pr) for pr in probs]
+    slots = [b.add(s, max_steps=30) for s in solvers]
+    while b.advance() > 0:
+        pass
+    for slot, s, ref in zip(slots, solvers, alone):
+        assert b.status(slot)[1]["steps"] == 30
+        for x, y in zip(_fields(s), ref):
+            assert np.array_equal(x, y), float(np.max(np.abs(x - y)))
+    b.close()
+    for s in solvers:
+        s.close()
 
 
 def test_batch_matches_alone_with_deep_water_strips():

@@ -22,6 +22,7 @@
 
 #include "../../include/thysea.h"
 #include "thy_aux.hpp"
+#include "thy_pool.hpp"
 #include "thy_sweep.hpp"
 
 using namespace thy;
@@ -61,6 +62,10 @@ struct thy_ctx {
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
     int order_n = 0;
     std::vector<int> order_h, cls_h;  // host copies: every block costliest first, and the classes
+    // host lists for batched runs, strips of srows rows: the deep-water ones, and the
+    // general ones with their cost classes; the general strips also cut into strips of
+    // gsrows rows (costliest first) with their classes
+    std::vector<int> bdeep_h, bgen_h, bgcls_h, bsub_h, bsubcls_h;
     int n_order = 0;             // blocks in `order` (the general launch's share)
     int *deep = nullptr;         // deep-water strips (n_deep of them)
     int *gen = nullptr;          // general strips beside them (n_gen)
@@ -300,7 +305,7 @@ int pick_tile(thy_ctx *c) {
 // kernel.  The estimate uses the still-water depth: a row counts as coastal for a
 // 64-column segment when any cell of it is shallower than 3 hpos (or dry).
 template <typename T>
-int build_order(thy_ctx *ctx, const std::vector<T> &D) {
+int build_order(thy_ctx *ctx, const T *D) {
     thy_ctx *c = ctx;
     if (c->kernel == 1) return THY_OK;
     const int nx = c->nx, ny = c->ny, P = c->pitch, S = c->srows;
@@ -365,6 +370,8 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
     const int G = (S > 8 && S % 8 == 0) ? 8 : S;
     c->gsrows = G;
     std::vector<int> deep, gen, gcls;
+    c->bgen_h.clear();
+    c->bgcls_h.clear();
     for (int t : base) {
         const int ty = t / tiles_x, tx = t - ty * tiles_x;
         for (int w = 0; w < SW_WAVES; w++) {
@@ -375,6 +382,8 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
                 deep.push_back(item);
                 continue;
             }
+            c->bgen_h.push_back(item);
+            c->bgcls_h.push_back(scls[item]);
             for (int sub = 0; sub < S / G && (ty * S + sub * G) < ny; sub++) {
                 gen.push_back((ty * (S / G) + sub) * nseg + sgi);
                 gcls.push_back(scls[item]);
@@ -385,13 +394,19 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
         std::vector<int> idx(gen.size());
         for (size_t i = 0; i < idx.size(); i++) idx[i] = (int)i;
         std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return gcls[a] > gcls[b]; });
-        std::vector<int> g2(gen.size());
-        for (size_t i = 0; i < idx.size(); i++) g2[i] = gen[idx[i]];
+        std::vector<int> g2(gen.size()), c2(gen.size());
+        for (size_t i = 0; i < idx.size(); i++) {
+            g2[i] = gen[idx[i]];
+            c2[i] = gcls[idx[i]];
+        }
         gen.swap(g2);
+        c->bsub_h = gen;
+        c->bsubcls_h.swap(c2);
     }
     std::stable_sort(base.begin(), base.end(), [&](int a, int b) { return cls[a] > cls[b]; });
     c->order_h = base;
     c->cls_h = cls;
+    c->bdeep_h = deep;
     if (c->gexec) {  // a captured step sequence holds the old tables and grid sizes
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
@@ -401,15 +416,15 @@ int build_order(thy_ctx *ctx, const std::vector<T> &D) {
     if (c->list_cap < cap) {
         void *ptrs[] = {c->order, c->deep, c->gen, c->fb_list};
         for (void *p : ptrs)
-            if (p) HIPCHK(hipFree(p));
+            dev_free(p);
         c->order = c->deep = c->gen = c->fb_list = nullptr;
-        HIPCHK(hipMalloc(&c->order, (size_t)cap * sizeof(int)));
-        HIPCHK(hipMalloc(&c->deep, (size_t)cap * sizeof(int)));
-        HIPCHK(hipMalloc(&c->gen, (size_t)cap * sizeof(int)));
-        HIPCHK(hipMalloc(&c->fb_list, (size_t)3 * cap * sizeof(int)));
+        HIPCHK(dev_alloc(&c->order, (size_t)cap * sizeof(int)));
+        HIPCHK(dev_alloc(&c->deep, (size_t)cap * sizeof(int)));
+        HIPCHK(dev_alloc(&c->gen, (size_t)cap * sizeof(int)));
+        HIPCHK(dev_alloc(&c->fb_list, (size_t)3 * cap * sizeof(int)));
         c->list_cap = cap;
     }
-    if (!c->fb_count) HIPCHK(hipMalloc(&c->fb_count, 3 * sizeof(unsigned int)));
+    if (!c->fb_count) HIPCHK(dev_alloc(&c->fb_count, 3 * sizeof(unsigned int)));
     HIPCHK(hipMemset(c->fb_count, 0, 3 * sizeof(unsigned int)));
     c->n_order = n_tiles;
     c->n_deep = (int)deep.size();
@@ -425,17 +440,30 @@ template <typename T>
 int rebuild_order_t(thy_ctx *ctx) {
     std::vector<T> D((size_t)ctx->pitch * ctx->ny);
     HIPCHK(hipMemcpy(D.data(), ctx->D, D.size() * sizeof(T), hipMemcpyDeviceToHost));
-    return build_order<T>(ctx, D);
+    return build_order<T>(ctx, D.data());
 }
 
 template <typename T>
 int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
     const double hmin = ctx->prm.hmin;
-    std::vector<T> e((size_t)P * ny, T(0)), D((size_t)P * ny, T(0)), qx((size_t)P * ny, T(0)), qy((size_t)P * ny, T(0));
+    const size_t n = (size_t)P * ny, b = n * sizeof(T);
+    // e, D, qx, qy staged in pinned host memory (a cached block), copied asynchronously
+    T *st = nullptr;
+    HIPCHK(Pool::get().host_alloc((void **)&st, 4 * b));
+    struct Staging {
+        T *p;
+        hipStream_t s;
+        ~Staging() {
+            (void)hipStreamSynchronize(s);  // the copies read it until they finish
+            Pool::get().host_free(p);
+        }
+    } staging{st, ctx->stream};
+    T *e = st, *D = st + n, *qx = st + 2 * n, *qy = st + 3 * n;
     for (int j = 0; j < ny; j++) {
+        const size_t r = (size_t)j * nx, ro = (size_t)j * P;
         for (int i = 0; i < nx; i++) {
-            const size_t c = (size_t)j * nx + i, o = (size_t)j * P + i;
+            const size_t c = r + i, o = ro + i;
             const double h = s1[2 * c], Hs = s1[2 * c + 1];
             const double Dd = Hs + hmin;             // unshifted depth
             D[o] = (T)Dd;
@@ -443,17 +471,20 @@ int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
             qx[o] = (T)s2[2 * c];
             qy[o] = (T)s2[2 * c + 1];
         }
+        for (int i = nx; i < P; i++) e[ro + i] = D[ro + i] = qx[ro + i] = qy[ro + i] = T(0);
     }
-    const size_t b = (size_t)P * ny * sizeof(T);
-    HIPCHK(hipMemcpy(ctx->e[0], e.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->e[1], e.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->D, D.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->qx[0], qx.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->qy[0], qy.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->qx[1], qx.data(), b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->qy[1], qy.data(), b, hipMemcpyHostToDevice));
+    hipStream_t q = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->e[0], e, b, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->D, D, b, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->qx[0], qx, b, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->qy[0], qy, b, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->e[1], ctx->e[0], b, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->qx[1], ctx->qx[0], b, hipMemcpyDeviceToDevice, q));
+    HIPCHK(hipMemcpyAsync(ctx->qy[1], ctx->qy[0], b, hipMemcpyDeviceToDevice, q));
     ctx->cur = 0;
-    return build_order<T>(ctx, D);
+    const int r = build_order<T>(ctx, D);  // on the host while the copies run
+    HIPCHK(hipStreamSynchronize(q));
+    return r;
 }
 
 template <typename T>
@@ -761,32 +792,31 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     int r = pick_tile(ctx);
     if (r != THY_OK) return r;
     HIPCHK(hipSetDevice(0));
-    HIPCHK(hipStreamCreate(&ctx->stream));
-    HIPCHK(hipStreamCreate(&ctx->side));
+    HIPCHK(Pool::get().get_streams(ctx->stream, ctx->side));
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
     for (int k = 0; k < 2; k++) {
-        HIPCHK(hipMalloc(&ctx->e[k], b));
-        HIPCHK(hipMalloc(&ctx->qx[k], b));
-        HIPCHK(hipMalloc(&ctx->qy[k], b));
+        HIPCHK(dev_alloc(&ctx->e[k], b));
+        HIPCHK(dev_alloc(&ctx->qx[k], b));
+        HIPCHK(dev_alloc(&ctx->qy[k], b));
     }
-    HIPCHK(hipMalloc(&ctx->D, b));
-    HIPCHK(hipMalloc(&ctx->emax, b));
+    HIPCHK(dev_alloc(&ctx->D, b));
+    HIPCHK(dev_alloc(&ctx->emax, b));
     if (p->track_arrival) {
-        HIPCHK(hipMalloc(&ctx->tarr, b));
-        HIPCHK(hipMalloc(&ctx->eini, b));
+        HIPCHK(dev_alloc(&ctx->tarr, b));
+        HIPCHK(dev_alloc(&ctx->eini, b));
     }
-    HIPCHK(hipMalloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
-    HIPCHK(hipMalloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
-    HIPCHK(hipMalloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
-    HIPCHK(hipMalloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
-    HIPCHK(hipMalloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
-    HIPCHK(hipMalloc((void **)&ctx->mass_partial, kMassBlocks * sizeof(double)));
-    HIPCHK(hipMalloc((void **)&ctx->steplog, (size_t)ctx->logcap * 2 * sizeof(double)));
-    HIPCHK(hipMalloc((void **)&ctx->area_d, ctx->ny * sizeof(double)));
+    HIPCHK(dev_alloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
+    HIPCHK(dev_alloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
+    HIPCHK(dev_alloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
+    HIPCHK(dev_alloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
+    HIPCHK(dev_alloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
+    HIPCHK(dev_alloc((void **)&ctx->mass_partial, kMassBlocks * sizeof(double)));
+    HIPCHK(dev_alloc((void **)&ctx->steplog, (size_t)ctx->logcap * 2 * sizeof(double)));
+    HIPCHK(dev_alloc((void **)&ctx->area_d, ctx->ny * sizeof(double)));
     {
         // mass weight per row: area*cos(phi), the invariant of the discrete scheme
         std::vector<double> w(ctx->ny);
@@ -797,16 +827,16 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
         std::vector<RowGeo<double>> rows;
         std::vector<FaceGeoY<double>> fys;
         fill_geometry(ctx, rows, fys);
-        HIPCHK(hipMalloc(&ctx->row, rows.size() * sizeof(rows[0])));
-        HIPCHK(hipMalloc(&ctx->fy, fys.size() * sizeof(fys[0])));
+        HIPCHK(dev_alloc(&ctx->row, rows.size() * sizeof(rows[0])));
+        HIPCHK(dev_alloc(&ctx->fy, fys.size() * sizeof(fys[0])));
         HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
     } else {
         std::vector<RowGeo<float>> rows;
         std::vector<FaceGeoY<float>> fys;
         fill_geometry(ctx, rows, fys);
-        HIPCHK(hipMalloc(&ctx->row, rows.size() * sizeof(rows[0])));
-        HIPCHK(hipMalloc(&ctx->fy, fys.size() * sizeof(fys[0])));
+        HIPCHK(dev_alloc(&ctx->row, rows.size() * sizeof(rows[0])));
+        HIPCHK(dev_alloc(&ctx->fy, fys.size() * sizeof(fys[0])));
         HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
     }
@@ -826,19 +856,21 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
 
 void thy_destroy(thy_ctx *ctx) {
     if (!ctx) return;
+    // no queued work may still use what goes back to the caches
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
                     ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list};
     for (void *q : ptrs)
-        if (q) (void)hipFree(q);
+        dev_free(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) Pool::get().put_streams(ctx->stream, ctx->side);
     delete ctx;
 }
 
@@ -1115,13 +1147,13 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
     ctx->nsnap = 0;
     ctx->logged = 0;
     const std::vector<double> &ft = ctx->ftimes;
-    if (ctx->ftime) { (void)hipFree(ctx->ftime); ctx->ftime = nullptr; }
-    HIPCHK(hipMalloc((void **)&ctx->ftime, ft.size() * sizeof(double)));
+    if (ctx->ftime) { dev_free(ctx->ftime); ctx->ftime = nullptr; }
+    HIPCHK(dev_alloc((void **)&ctx->ftime, ft.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->ftime, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
     // gauges
-    if (ctx->gpos) { (void)hipFree(ctx->gpos); ctx->gpos = nullptr; }
-    if (ctx->gbuf) { (void)hipFree(ctx->gbuf); ctx->gbuf = nullptr; }
-    if (ctx->gtime) { (void)hipFree(ctx->gtime); ctx->gtime = nullptr; }
+    if (ctx->gpos) { dev_free(ctx->gpos); ctx->gpos = nullptr; }
+    if (ctx->gbuf) { dev_free(ctx->gbuf); ctx->gbuf = nullptr; }
+    if (ctx->gtime) { dev_free(ctx->gtime); ctx->gtime = nullptr; }
     ctx->ng = cfg->n_gauges;
     ctx->gcap = 0;
     if (cfg->n_gauges > 0 && cfg->series_interval >= 0.0) {
@@ -1129,10 +1161,10 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
         if (cfg->max_steps > 0 && cfg->series_interval <= 0.0) nsd = (double)cfg->max_steps + 2.0;
         ctx->gcap = (int)std::min(nsd, 1e6);
     }
-    HIPCHK(hipMalloc((void **)&ctx->gpos, std::max(1, ctx->ng) * sizeof(int)));
+    HIPCHK(dev_alloc((void **)&ctx->gpos, std::max(1, ctx->ng) * sizeof(int)));
     if (ctx->ng) HIPCHK(hipMemcpy(ctx->gpos, cfg->gauge_pos, ctx->ng * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc((void **)&ctx->gbuf, std::max<size_t>(1, (size_t)ctx->gcap * ctx->ng * 3) * sizeof(float)));
-    HIPCHK(hipMalloc((void **)&ctx->gtime, std::max(1, ctx->gcap) * sizeof(float)));
+    HIPCHK(dev_alloc((void **)&ctx->gbuf, std::max<size_t>(1, (size_t)ctx->gcap * ctx->ng * 3) * sizeof(float)));
+    HIPCHK(dev_alloc((void **)&ctx->gtime, std::max(1, ctx->gcap) * sizeof(float)));
     if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; ctx->gk = 0; }
 
     // src/GPU/ShallowWater.cu:305-309: metrics from the pre-deformation state
@@ -1203,29 +1235,41 @@ int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res) {
 }  // extern "C"
 
 // ===========================================================================
-// Batched scenarios: many contexts advanced by one persistent launch per step.
+// Batched scenarios: many contexts advanced together, three launches per step (the
+// deep-water strips of every scenario, the general strips beside them on a second
+// stream, then the hand-backs), captured once for K steps.
 // ===========================================================================
 struct thy_batch {
     int capacity = 0;
     size_t budget = 0, used = 0;
     int K = 24;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_done = nullptr;   // end of a launch's work (admissions queue copies after it)
     std::vector<thy_ctx *> ctx;     // per slot; nullptr when free
     std::vector<size_t> bytes;      // admitted footprint per slot
+    std::vector<int> srows;         // strip height in the slot's argument block
     std::vector<int> state;         // THY_SLOT_*
     std::vector<thy_run_result> res;
     StepArgs<float> *d_args = nullptr;
-    int2 *d_items = nullptr;
-    int items_cap = 0;
-    int *d_nitems = nullptr;
-    unsigned int *d_counter = nullptr;
+    // item lists (scenario slot, strip << 1 | writer) and their device counts; the
+    // captured grids cover deep_cap and gen_cap items
+    int2 *d_deep = nullptr, *d_gen = nullptr, *d_fb = nullptr;
+    int *d_counts = nullptr;        // [0] deep items, [1] general items
+    unsigned int *d_fb_count = nullptr;
+    int deep_cap = 0, gen_cap = 0;
+    int gen_sub = 0;                // general items are gsrows-row sub-strips
+    std::vector<int2> h_deep, h_gen;
+    int h_counts[2] = {0, 0};
+    int fb_grid = 1;
     Ctrl *d_ctrl = nullptr;
-    std::vector<Ctrl> h_ctrl;
+    Ctrl *h_ctrl = nullptr;              // pinned: control blocks gathered after a launch
+    StepArgs<float> *h_args = nullptr;   // pinned: argument blocks of admitted scenarios
     hipGraphExec_t gexec = nullptr;
-    int grid = 0;
-    int n_items = 0;
     bool dirty = true;
+    bool in_flight = false;              // a launch is queued and not yet collected
+    std::vector<char> launched;          // per slot: running when the launch was queued
+    int launched_hi = 0;
     double gpu_ms = 0.0;
     std::string err;
 };
@@ -1245,52 +1289,109 @@ int bfail(thy_batch *b, int code, const std::string &msg) {
                          std::string(#x) + ": " + hipGetErrorString(e_));                                \
     } while (0)
 
-// Work items of the running scenarios: costliest cost class first, then by scenario
-// and the scenario's own dispatch order.
+// capacity of an item list: a power of two, at least 256 items
+int list_cap(size_t n) {
+    int c = 256;
+    while ((size_t)c < n) c <<= 1;
+    return c;
+}
+
+// Item lists of the running scenarios: every deep-water strip, scenario by scenario in
+// its own order (neighbouring strips stay together); every general strip, costliest
+// cost class first.  A scenario's writer is its first deep-water strip, or its first
+// general strip when it has none.  With few general strips in all (a launch bound by
+// the time one wavefront takes for one strip rather than by the number of strips) the
+// general strips are cut into sub-strips of each scenario's gsrows rows, as in a run
+// alone.
 int batch_items(thy_batch *b) {
-    struct It { int cls, sc, pos, tile; };
-    std::vector<It> v;
+    struct G { int cls; int2 it; };
+    std::vector<G> gv;
+    b->h_deep.clear();
+    size_t n_full = 0;
+    for (int i = 0; i < b->capacity; i++)
+        if (b->state[i] == THY_SLOT_RUNNING) n_full += b->ctx[i]->bgen_h.size();
+    const int sub = n_full < (size_t)8 * b->fb_grid ? 1 : 0;
     for (int i = 0; i < b->capacity; i++) {
         if (b->state[i] != THY_SLOT_RUNNING) continue;
-        thy_ctx *c = b->ctx[i];
-        for (size_t p = 0; p < c->order_h.size(); p++) {
-            const int t = c->order_h[p];
-            v.push_back({c->cls_h[t], i, (int)p, t});
-        }
+        const thy_ctx *c = b->ctx[i];
+        // the strip lists must describe the strips of the argument block
+        if (c->srows != b->srows[i]) return bfail(b, THY_ERR_INVALID, "strip height changed after admission");
+        const bool nd = c->bdeep_h.empty();
+        for (size_t p = 0; p < c->bdeep_h.size(); p++)
+            b->h_deep.push_back(make_int2(i, (c->bdeep_h[p] << 1) | (p == 0 ? 1 : 0)));
+        const std::vector<int> &gl = sub ? c->bsub_h : c->bgen_h, &gc = sub ? c->bsubcls_h : c->bgcls_h;
+        for (size_t p = 0; p < gl.size(); p++)
+            gv.push_back({gc[p], make_int2(i, (gl[p] << 1) | (nd && p == 0 ? 1 : 0))});
     }
-    std::stable_sort(v.begin(), v.end(), [](const It &x, const It &y) { return x.cls > y.cls; });
-    std::vector<int2> items(v.size());
-    for (size_t k = 0; k < v.size(); k++) items[k] = make_int2(v[k].sc, v[k].tile);
-    if ((int)items.size() > b->items_cap) {
-        if (b->d_items) BHIPCHK(hipFree(b->d_items));
-        b->d_items = nullptr;
-        b->items_cap = std::max<int>((int)items.size(), 2 * b->items_cap);
-        BHIPCHK(hipMalloc(&b->d_items, (size_t)b->items_cap * sizeof(int2)));
-        if (b->gexec) {  // the captured launches hold the old item array
+    std::stable_sort(gv.begin(), gv.end(), [](const G &x, const G &y) { return x.cls > y.cls; });
+    b->h_gen.resize(gv.size());
+    for (size_t k = 0; k < gv.size(); k++) b->h_gen[k] = gv[k].it;
+    const int dc = list_cap(b->h_deep.size()), gc = list_cap(b->h_gen.size());
+    if (sub != b->gen_sub && b->gexec) {  // captured with the other strip height
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    b->gen_sub = sub;
+    if (dc != b->deep_cap || gc != b->gen_cap) {
+        // the captured launches hold the old lists and grid sizes
+        if (b->gexec) {
             (void)hipGraphExecDestroy(b->gexec);
             b->gexec = nullptr;
         }
+        void *ptrs[] = {b->d_deep, b->d_gen, b->d_fb};
+        for (void *q : ptrs)
+            dev_free(q);
+        b->d_deep = b->d_gen = b->d_fb = nullptr;
+        b->deep_cap = b->gen_cap = 0;
+        BHIPCHK(dev_alloc(&b->d_deep, (size_t)dc * sizeof(int2)));
+        BHIPCHK(dev_alloc(&b->d_gen, (size_t)gc * sizeof(int2)));
+        BHIPCHK(dev_alloc(&b->d_fb, (size_t)3 * dc * sizeof(int2)));
+        b->deep_cap = dc;
+        b->gen_cap = gc;
     }
-    b->n_items = (int)items.size();
-    if (!items.empty())
-        BHIPCHK(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
-    BHIPCHK(hipMemcpy(b->d_nitems, &b->n_items, sizeof(int), hipMemcpyHostToDevice));
+    b->h_counts[0] = (int)b->h_deep.size();
+    b->h_counts[1] = (int)b->h_gen.size();
+    // ordered before the next launch on the batch stream; the host lists live in b
+    if (!b->h_deep.empty())
+        BHIPCHK(hipMemcpyAsync(b->d_deep, b->h_deep.data(), b->h_deep.size() * sizeof(int2), hipMemcpyHostToDevice,
+                               b->stream));
+    if (!b->h_gen.empty())
+        BHIPCHK(hipMemcpyAsync(b->d_gen, b->h_gen.data(), b->h_gen.size() * sizeof(int2), hipMemcpyHostToDevice,
+                               b->stream));
+    BHIPCHK(hipMemcpyAsync(b->d_counts, b->h_counts, sizeof(b->h_counts), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(hipStreamSynchronize(b->stream));
     b->dirty = false;
     return THY_OK;
 }
 
-// K launches (slots 0, 1, 2, ...) of the persistent step kernel, captured once
+// K steps (launch slots 0, 1, 2, ...), captured once: per step the deep-water launch on
+// the batch stream, the general launch beside it on the side stream, and after both
+// the hand-back launch
 int batch_graph(thy_batch *b) {
     if (b->gexec) return THY_OK;
-    QueueArgs q;
+    BatchArgs q;
     q.args = b->d_args;
-    q.items = b->d_items;
-    q.n_items = b->d_nitems;
-    q.counter = b->d_counter;
+    q.deep = b->d_deep;
+    q.n_deep = b->d_counts;
+    q.gen = b->d_gen;
+    q.n_gen = b->d_counts + 1;
+    q.fb_list = b->d_fb;
+    q.fb_count = b->d_fb_count;
+    q.fb_cap = b->deep_cap;
+    q.gen_sub = b->gen_sub;
+    const dim3 blk(64 * SW_WAVES);
     hipGraph_t g;
     BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < b->K; k++)
-        hipLaunchKernelGGL(k_sweep_queue, dim3(b->grid), dim3(64 * SW_WAVES), 0, b->stream, q, k % 3);
+    for (int k = 0; k < b->K; k++) {
+        const int slot = k % 3;
+        BHIPCHK(hipEventRecord(b->ev_fork, b->stream));
+        BHIPCHK(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+        hipLaunchKernelGGL(k_batch_general<false>, dim3(b->gen_cap / SW_WAVES), blk, 0, b->side, q, slot);
+        BHIPCHK(hipEventRecord(b->ev_join, b->side));
+        hipLaunchKernelGGL(k_batch_deep, dim3(b->deep_cap / SW_WAVES), blk, 0, b->stream, q, slot);
+        BHIPCHK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+        hipLaunchKernelGGL(k_batch_general<true>, dim3(b->fb_grid), blk, 0, b->stream, q, slot);
+    }
     BHIPCHK(hipStreamEndCapture(b->stream, &g));
     BHIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     (void)hipGraphDestroy(g);
@@ -1329,37 +1430,47 @@ int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_ba
     b->K = steps_per_sync > 0 ? ((steps_per_sync + 2) / 3) * 3 : 24;
     b->ctx.assign(capacity, nullptr);
     b->bytes.assign(capacity, 0);
+    b->srows.assign(capacity, 0);
     b->state.assign(capacity, THY_SLOT_FREE);
     b->res.assign(capacity, thy_run_result{});
-    b->h_ctrl.resize(capacity);
+    b->launched.assign(capacity, 0);
     BHIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BHIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     BHIPCHK(hipEventCreate(&b->ev0));
     BHIPCHK(hipEventCreate(&b->ev1));
-    BHIPCHK(hipMalloc(&b->d_args, (size_t)capacity * sizeof(StepArgs<float>)));
+    BHIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    BHIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    BHIPCHK(hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming));
+    BHIPCHK(dev_alloc(&b->d_args, (size_t)capacity * sizeof(StepArgs<float>)));
     BHIPCHK(hipMemset(b->d_args, 0, (size_t)capacity * sizeof(StepArgs<float>)));
-    BHIPCHK(hipMalloc(&b->d_nitems, sizeof(int)));
-    BHIPCHK(hipMemset(b->d_nitems, 0, sizeof(int)));
-    BHIPCHK(hipMalloc(&b->d_counter, 3 * sizeof(unsigned int)));
-    BHIPCHK(hipMemset(b->d_counter, 0, 3 * sizeof(unsigned int)));
-    BHIPCHK(hipMalloc(&b->d_ctrl, (size_t)capacity * sizeof(Ctrl)));
-    b->items_cap = 1024;
-    BHIPCHK(hipMalloc(&b->d_items, (size_t)b->items_cap * sizeof(int2)));
-    int dev = 0, ncu = 0, per_cu = 0;
+    BHIPCHK(dev_alloc(&b->d_counts, 2 * sizeof(int)));
+    BHIPCHK(hipMemset(b->d_counts, 0, 2 * sizeof(int)));
+    BHIPCHK(dev_alloc(&b->d_fb_count, 3 * sizeof(unsigned int)));
+    BHIPCHK(hipMemset(b->d_fb_count, 0, 3 * sizeof(unsigned int)));
+    BHIPCHK(dev_alloc(&b->d_ctrl, (size_t)capacity * sizeof(Ctrl)));
+    BHIPCHK(Pool::get().host_alloc((void **)&b->h_ctrl, (size_t)capacity * sizeof(Ctrl)));
+    BHIPCHK(Pool::get().host_alloc((void **)&b->h_args, (size_t)capacity * sizeof(StepArgs<float>)));
+    int dev = 0, ncu = 0;
     BHIPCHK(hipGetDevice(&dev));
     BHIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_queue, 64 * SW_WAVES, 0));
-    b->grid = std::max(1, ncu * std::max(1, per_cu));
+    b->fb_grid = std::max(1, ncu);
     return THY_OK;
 }
 
 void thy_batch_destroy(thy_batch *b) {
     if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->side) (void)hipStreamSynchronize(b->side);
+    Pool::get().host_free(b->h_ctrl);
+    Pool::get().host_free(b->h_args);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    void *ptrs[] = {b->d_args, b->d_items, b->d_nitems, b->d_counter, b->d_ctrl};
+    void *ptrs[] = {b->d_args, b->d_deep, b->d_gen, b->d_fb, b->d_counts, b->d_fb_count, b->d_ctrl};
     for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    if (b->ev0) (void)hipEventDestroy(b->ev0);
-    if (b->ev1) (void)hipEventDestroy(b->ev1);
+        dev_free(p);
+    hipEvent_t evs[] = {b->ev0, b->ev1, b->ev_fork, b->ev_join, b->ev_done};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1387,9 +1498,12 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
     if ((r = read_ctrl(ctx, &cb)) != THY_OK) return bfail(b, r, thy_last_error(ctx));
     bool changed;
     if ((r = batch_faults(b, ctx, cb, changed)) != THY_OK) return r;
-    StepArgs<float> a = make_args<float>(ctx);
-    BHIPCHK(hipMemcpy(b->d_args + slot, &a, sizeof(a), hipMemcpyHostToDevice));
+    const StepArgs<float> a = make_args<float>(ctx);
+    // ordered after a launch in flight, which does not read this free slot
+    b->h_args[slot] = a;
+    BHIPCHK(hipMemcpyAsync(b->d_args + slot, b->h_args + slot, sizeof(a), hipMemcpyHostToDevice, b->stream));
     b->ctx[slot] = ctx;
+    b->srows[slot] = a.srows;
     b->bytes[slot] = need;
     b->used += need;
     b->state[slot] = THY_SLOT_RUNNING;
@@ -1399,18 +1513,17 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
     return THY_OK;
 }
 
-int thy_batch_advance(thy_batch *b, int *n_running) {
+int thy_batch_launch(thy_batch *b) {
     if (!b) return THY_ERR_INVALID;
+    if (b->in_flight) return bfail(b, THY_ERR_INVALID, "a launch is in flight");
     int r;
     if (b->dirty && (r = batch_items(b)) != THY_OK) return r;
-    int hi = 0, running = 0;
-    for (int i = 0; i < b->capacity; i++)
-        if (b->state[i] == THY_SLOT_RUNNING) {
-            hi = i + 1;
-            running++;
-        }
-    if (n_running) *n_running = running;
-    if (running == 0) return THY_OK;
+    int hi = 0;
+    for (int i = 0; i < b->capacity; i++) {
+        b->launched[i] = b->state[i] == THY_SLOT_RUNNING;
+        if (b->launched[i]) hi = i + 1;
+    }
+    if (hi == 0) return THY_OK;
     if ((r = batch_graph(b)) != THY_OK) return r;
     BHIPCHK(hipEventRecord(b->ev0, b->stream));
     BHIPCHK(hipGraphLaunch(b->gexec, b->stream));
@@ -1418,38 +1531,73 @@ int thy_batch_advance(thy_batch *b, int *n_running) {
     // K is a multiple of 3: the next launch reads slot 0 again
     hipLaunchKernelGGL(k_gather_ctrl, dim3((hi + 63) / 64), dim3(64), 0, b->stream, b->d_args, hi, 0, b->d_ctrl);
     BHIPCHK(hipGetLastError());
-    BHIPCHK(hipMemcpyAsync(b->h_ctrl.data(), b->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost, b->stream));
-    BHIPCHK(hipStreamSynchronize(b->stream));
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
-    b->gpu_ms += ms;
-    for (int i = 0; i < hi; i++) {
-        if (b->state[i] != THY_SLOT_RUNNING) continue;
-        thy_ctx *c = b->ctx[i];
-        Ctrl cb = b->h_ctrl[i];
-        c->cur = cb.cur;
-        if (!cb.dt_valid) continue;  // stepped through the whole replay
-        // stopped: finished, or a fault is due (the only other stop a batched run has)
-        if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {
-            c->nsamples = std::min(cb.nsamples, c->gcap);
-            if ((r = collect_gauges(c)) != THY_OK) return bfail(b, r, thy_last_error(c));
-            thy_run_result &res = b->res[i];
-            res.steps = cb.steps;
-            res.t_final = cb.t;
-            res.n_samples = c->nsamples;
-            b->state[i] = THY_SLOT_FINISHED;
-            b->dirty = true;
-            running--;
-            // no launch or gather reads a finished scenario again
-            BHIPCHK(hipMemset(b->d_args + i, 0, sizeof(StepArgs<float>)));
-            continue;
-        }
-        bool changed;
-        if ((r = batch_faults(b, c, cb, changed)) != THY_OK) return r;
-        if (!changed) return bfail(b, THY_ERR_INVALID, "batched scenario stalled");
-    }
-    if (n_running) *n_running = running;
+    BHIPCHK(hipMemcpyAsync(b->h_ctrl, b->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(hipEventRecord(b->ev_done, b->stream));
+    b->launched_hi = hi;
+    b->in_flight = true;
     return THY_OK;
+}
+
+int thy_batch_ready(thy_batch *b, int *done) {
+    if (!b || !done) return THY_ERR_INVALID;
+    if (!b->in_flight) {
+        *done = 1;
+        return THY_OK;
+    }
+    // the launch's own work: copies that admissions queued after it do not count
+    const hipError_t e = hipEventQuery(b->ev_done);
+    if (e != hipSuccess && e != hipErrorNotReady) BHIPCHK(e);
+    *done = e == hipSuccess ? 1 : 0;
+    return THY_OK;
+}
+
+int thy_batch_wait(thy_batch *b, int *n_running) {
+    if (!b) return THY_ERR_INVALID;
+    int r;
+    if (b->in_flight) {
+        b->in_flight = false;
+        BHIPCHK(hipStreamSynchronize(b->stream));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        b->gpu_ms += ms;
+        for (int i = 0; i < b->launched_hi; i++) {
+            if (!b->launched[i] || b->state[i] != THY_SLOT_RUNNING) continue;
+            thy_ctx *c = b->ctx[i];
+            Ctrl cb = b->h_ctrl[i];
+            c->cur = cb.cur;
+            if (!cb.dt_valid) continue;  // stepped through the whole sequence
+            // stopped: finished, or a fault is due (the only other stop a batched run has)
+            if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {
+                c->nsamples = std::min(cb.nsamples, c->gcap);
+                if ((r = collect_gauges(c)) != THY_OK) return bfail(b, r, thy_last_error(c));
+                thy_run_result &res = b->res[i];
+                res.steps = cb.steps;
+                res.t_final = cb.t;
+                res.n_samples = c->nsamples;
+                b->state[i] = THY_SLOT_FINISHED;
+                b->dirty = true;
+                // no later launch or gather reads a finished scenario
+                BHIPCHK(hipMemsetAsync(b->d_args + i, 0, sizeof(StepArgs<float>), b->stream));
+                continue;
+            }
+            bool changed;
+            if ((r = batch_faults(b, c, cb, changed)) != THY_OK) return r;
+            if (!changed) return bfail(b, THY_ERR_INVALID, "batched scenario stalled");
+        }
+        BHIPCHK(hipStreamSynchronize(b->stream));
+    }
+    if (n_running) {
+        int running = 0;
+        for (int i = 0; i < b->capacity; i++) running += b->state[i] == THY_SLOT_RUNNING;
+        *n_running = running;
+    }
+    return THY_OK;
+}
+
+int thy_batch_advance(thy_batch *b, int *n_running) {
+    int r = thy_batch_launch(b);
+    if (r != THY_OK) return r;
+    return thy_batch_wait(b, n_running);
 }
 
 int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res) {
@@ -1461,8 +1609,12 @@ int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res) {
 
 int thy_batch_remove(thy_batch *b, int slot) {
     if (!b || slot < 0 || slot >= b->capacity || b->state[slot] == THY_SLOT_FREE) return THY_ERR_INVALID;
-    BHIPCHK(hipStreamSynchronize(b->stream));
-    BHIPCHK(hipMemset(b->d_args + slot, 0, sizeof(StepArgs<float>)));
+    if (b->state[slot] == THY_SLOT_RUNNING) {
+        // a launch in flight may be stepping it: wait, and take it out of every later one
+        BHIPCHK(hipStreamSynchronize(b->stream));
+        BHIPCHK(hipMemset(b->d_args + slot, 0, sizeof(StepArgs<float>)));
+    }
+    // a finished slot's argument block was cleared when it finished
     b->ctx[slot]->run_active = false;
     b->used -= b->bytes[slot];
     b->bytes[slot] = 0;
@@ -1471,6 +1623,8 @@ int thy_batch_remove(thy_batch *b, int slot) {
     b->dirty = true;
     return THY_OK;
 }
+
+void thy_release_cached(void) { Pool::get().trim(); }
 
 int thy_batch_info(thy_batch *b, int *running, size_t *bytes_used, double *gpu_seconds) {
     if (!b) return THY_ERR_INVALID;

@@ -704,45 +704,103 @@ namespace thy {
 
 // ---------------------------------------------------------------------------
 // Many grids per launch.  Each scenario keeps its own state, control block and dt
-// slots; a launch advances every scenario of the item list by one of its own steps.
-// Work items are (scenario, strip block) pairs, costliest first; a fixed set of
-// resident workgroups takes them from a device counter, so the launch balances
-// itself over scenarios of different sizes and finishing times, and the same
-// captured launch sequence serves a changing item list (the count is read on the
-// device).
+// slots; a step of the batch advances every running scenario by one of its own steps
+// in three launches, as for one grid: the deep-water strips of all scenarios, beside
+// them (on a second stream) the general strips of all scenarios, costliest first, and
+// after both the strips that the deep-water launch handed back.  Every wavefront takes
+// one item (scenario, strip) and runs the step prologue of its scenario itself; the
+// item flagged as its scenario's writer publishes the next control block.  Item
+// counts are read on the device, so one captured launch sequence serves a changing
+// item list up to the captured grid sizes.
 // ---------------------------------------------------------------------------
-struct QueueArgs {
-    const StepArgs<float> *args;  // one per scenario slot
-    const int2 *items;            // (scenario slot, strip block)
-    const int *n_items;
-    unsigned int *counter;        // 3 slots, launch n uses slot n mod 3
+struct BatchArgs {
+    const StepArgs<float> *args;  // per scenario slot (zeroed while the slot is free)
+    const int2 *deep;             // (scenario slot, strip << 1 | writer): deep-water strips
+    const int *n_deep;
+    const int2 *gen;              // (scenario slot, strip << 1 | writer): general strips
+    const int *n_gen;
+    int2 *fb_list;                // 3 slots of fb_cap hand-backs (scenario slot, strip)
+    unsigned int *fb_count;       // 3 slots: launch n appends to slot n mod 3
+    int fb_cap;
+    int gen_sub;                  // general items are strips of gsrows rows (else srows)
 };
 
-__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_queue(const QueueArgs q, const int slot) {
-    __shared__ BlockCtl<float> bc;
-    __shared__ float s_red[SW_WAVES];
-    __shared__ int s_item;
-    const int tid = threadIdx.x;
-    const int n_items = *q.n_items;
-    for (;;) {
-        if (tid == 0) {
-            const int it = (int)atomicAdd(&q.counter[slot], 1u);
-            s_item = it;
-            if (it == 0) q.counter[(slot + 1) % 3] = 0u;  // the next launch's counter
+// the step prologue of one wavefront's scenario (every lane computes the same values;
+// lane 0 of the writer item publishes the next control block and samples the gauges)
+__device__ __forceinline__ BlockCtl<float> wave_begin(const StepArgs<float> &a, const int slot, const bool writer) {
+    const int lane = threadIdx.x & 63;
+    BlockCtl<float> bc;
+    begin_step(a, slot, writer && lane == 0, bc);
+    if (writer && bc.run) {
+        const bool c1 = bc.cur != 0;
+        gather_gauges(a, bc, c1 ? a.e[1] : a.e[0], c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], lane, 64);
+    }
+    return bc;
+}
+
+__device__ __forceinline__ void wave_dt_min(const StepArgs<float> &a, const int slot, const float dtmin) {
+    const float m = wave_min(dtmin);
+    if ((threadIdx.x & 63) == 0) atomicMin(&a.dtmin[slot], __float_as_uint(m));
+}
+
+__global__ void __launch_bounds__(64 * SW_WAVES) k_batch_deep(const BatchArgs q, const int slot) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (k == 0 && lane == 0) q.fb_count[(slot + 1) % 3] = 0u;  // the next launch's hand-back list
+    const int n = __builtin_amdgcn_readfirstlane(*q.n_deep);
+    if (k >= n) return;
+    const int2 w = q.deep[k];
+    const int sc = __builtin_amdgcn_readfirstlane(w.x), code = __builtin_amdgcn_readfirstlane(w.y);
+    const StepArgs<float> a = uniform_load(q.args + sc);
+    if (!a.ctrl) return;
+    const BlockCtl<float> bc = wave_begin(a, slot, (code & 1) != 0);
+    if (!bc.run) return;
+    int x0, y0;
+    item_origin(a, code >> 1, a.srows, x0, y0);
+    float dtmin;
+    if (!deep_strip(a, bc, x0, y0, dtmin)) {
+        // hand the strip back; its dt comes from the launch that redoes it
+        dtmin = 3.0e38f;
+        if (lane == 0) {
+            const unsigned m = atomicAdd(&q.fb_count[slot], 1u);
+            if (m < (unsigned)q.fb_cap) q.fb_list[slot * q.fb_cap + m] = make_int2(sc, code >> 1);
         }
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        if (it >= n_items) break;
-        const int2 w = q.items[it];
-        const int sc = __builtin_amdgcn_readfirstlane(w.x), tile = __builtin_amdgcn_readfirstlane(w.y);
-        const StepArgs<float> a = uniform_load(q.args + sc);
-        if (tid == 0) begin_step(a, slot, tile == 0, bc);
-        __syncthreads();
-        if (bc.run) {
-            const float dtmin = sweep_block(a, bc, tile, tile == 0);
-            block_dt_min(a, slot, dtmin, s_red);
+    }
+    wave_dt_min(a, slot, dtmin);
+}
+
+// one wavefront's general strip of the batch (item code: strip << 1 | writer; strips
+// of gsrows rows when sub, else of srows rows)
+__device__ __forceinline__ void batch_general_item(const BatchArgs &q, const int slot, const int sc, const int code,
+                                                   const bool sub) {
+    const StepArgs<float> a = uniform_load(q.args + sc);
+    if (!a.ctrl) return;
+    const BlockCtl<float> bc = wave_begin(a, slot, (code & 1) != 0);
+    if (!bc.run) return;
+    const int rows = sub ? a.gsrows : a.srows;
+    int x0, y0;
+    item_origin(a, code >> 1, rows, x0, y0);
+    wave_dt_min(a, slot, sweep_strip(a, bc, x0, y0, rows));
+}
+
+// HANDBACK false: the general strips, one wavefront each.  HANDBACK true: the strips
+// that this step's deep-water launch handed back; a small grid strides over them.
+template <bool HANDBACK>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_batch_general(const BatchArgs q, const int slot) {
+    const int k0 = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (!HANDBACK) {
+        const int n = __builtin_amdgcn_readfirstlane(*q.n_gen);
+        if (k0 >= n) return;
+        const int2 w = q.gen[k0];
+        batch_general_item(q, slot, __builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y),
+                           q.gen_sub != 0);
+    } else {
+        const int n = __builtin_amdgcn_readfirstlane((int)min(q.fb_count[slot], (unsigned)q.fb_cap));
+        for (int k = k0; k < n; k += gridDim.x * SW_WAVES) {
+            const int2 w = q.fb_list[slot * q.fb_cap + k];
+            batch_general_item(q, slot, __builtin_amdgcn_readfirstlane(w.x),
+                               __builtin_amdgcn_readfirstlane(w.y) << 1, false);
         }
-        __syncthreads();  // bc, s_red and s_item are reused by the next item
     }
 }
 

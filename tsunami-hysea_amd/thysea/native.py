@@ -104,6 +104,11 @@ def lib():
         L.thy_batch_last_error.restype = C.c_char_p
         L.thy_batch_add.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), C.POINTER(C.c_int)]
         L.thy_batch_advance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.thy_batch_launch.argtypes = [C.c_void_p]
+        L.thy_batch_ready.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.thy_batch_wait.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.thy_release_cached.argtypes = []
+        L.thy_release_cached.restype = None
         L.thy_batch_status.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(RunResult)]
         L.thy_batch_remove.argtypes = [C.c_void_p, C.c_int]
         L.thy_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
@@ -113,6 +118,12 @@ def lib():
 
 def probe():
     return lib().thy_probe()
+
+
+def release_cached():
+    """Release the device memory, pinned memory and streams kept in the library's caches
+    (call with no Solver or Batch open)."""
+    lib().thy_release_cached()
 
 
 def params_of(prob, precision=0, track_arrival=True, tile=(0, 0), kernel="sweep", strip_rows=0):
@@ -304,9 +315,9 @@ class Batch:
             msg = lib().thy_batch_last_error(self.h).decode(errors="replace")
             raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {msg}", rc)
 
-    def add(self, solver, steps_per_sync=0):
+    def add(self, solver, steps_per_sync=0, max_steps=0):
         """Admit a scenario; returns its slot, or None when the budget or the slots are full."""
-        c = solver.run_config(steps_per_sync=steps_per_sync, save_interval=-1.0)
+        c = solver.run_config(max_steps=max_steps, steps_per_sync=steps_per_sync, save_interval=-1.0)
         slot = C.c_int()
         rc = lib().thy_batch_add(self.h, solver.h, C.byref(c), C.byref(slot))
         if rc == STATUS_CODES["BUDGET"]:
@@ -316,8 +327,25 @@ class Batch:
         return slot.value
 
     def advance(self):
+        """One step sequence for every running scenario; returns how many still run."""
         n = C.c_int()
         self._chk(lib().thy_batch_advance(self.h, C.byref(n)), "batch_advance")
+        return n.value
+
+    def launch(self):
+        """Queue one step sequence and return at once (collect it with wait())."""
+        self._chk(lib().thy_batch_launch(self.h), "batch_launch")
+
+    def ready(self):
+        """True when no launch is in flight or it has finished."""
+        d = C.c_int()
+        self._chk(lib().thy_batch_ready(self.h, C.byref(d)), "batch_ready")
+        return bool(d.value)
+
+    def wait(self):
+        """Collect the launch in flight; returns how many scenarios still run."""
+        n = C.c_int()
+        self._chk(lib().thy_batch_wait(self.h, C.byref(n)), "batch_wait")
         return n.value
 
     def status(self, slot):

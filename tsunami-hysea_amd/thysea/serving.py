@@ -19,9 +19,15 @@ def _result(r):
             "steps": r["steps"]}
 
 
-def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None):
+def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8):
     """Serve ``problems`` (arrival times in seconds, default all at 0) through one
-    Batch.  Returns per-request records and, for indices in ``keep``, the results."""
+    Batch.  Returns per-request records and, for indices in ``keep``, the results.
+
+    The host work of a request (context set-up, upload, admission; read-out and
+    release once finished) runs while the GPU steps the batch: each step sequence is
+    launched asynchronously and the host works through finished and arrived requests
+    until it has completed (at most ``burst`` admissions while the GPU is idle, so
+    that stepping starts early)."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
     order = list(np.argsort(arrivals, kind="stable"))
@@ -29,36 +35,63 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     rec = [dict(arrival=float(arrivals[i]), admitted=None, finished=None, steps=0) for i in range(n)]
     results = {}
     b = native.Batch(capacity, hbm_budget=hbm_budget, steps_per_sync=steps_per_sync)
-    live = {}  # slot -> (request index, solver)
+    live = {}      # slot -> (request index, solver)
+    pending = []   # finished slots not yet read out
     t0 = time.perf_counter()
     nxt = 0
     done = 0
+    in_flight = False
+
+    def admit():
+        i = order[nxt]
+        s = native.Solver(problems[i])
+        slot = b.add(s, steps_per_sync=steps_per_sync)
+        if slot is None:  # budget or slots full: wait for a finisher
+            s.close()
+            if not live:
+                raise native.ThyError(f"request {i} does not fit the batch's HBM budget")
+            return False
+        live[slot] = (i, s)
+        rec[i]["admitted"] = time.perf_counter() - t0
+        return True
+
+    def retire(slot):
+        i, s = live.pop(slot)
+        r = b.result(slot)
+        rec[i]["finished"] = time.perf_counter() - t0
+        rec[i]["steps"] = int(r["steps"])
+        if i in keep:
+            results[i] = _result(r)
+        b.remove(slot)
+        s.close()
+
     while done < n:
-        now = time.perf_counter() - t0
-        # admit every arrived request that fits
-        while nxt < n and arrivals[order[nxt]] <= now:
-            i = order[nxt]
-            s = native.Solver(problems[i])
-            slot = b.add(s, steps_per_sync=steps_per_sync)
-            if slot is None:  # budget or slots full: wait for a finisher
-                s.close()
+        k = 0
+        while True:
+            if in_flight:
+                if b.ready():
+                    break
+            elif k >= burst:
                 break
-            live[slot] = (i, s)
-            rec[i]["admitted"] = time.perf_counter() - t0
-            nxt += 1
-        if live:
-            b.advance()
-            for slot in b.finished():
-                i, s = live.pop(slot)
-                r = b.result(slot)
-                rec[i]["finished"] = time.perf_counter() - t0
-                rec[i]["steps"] = int(r["steps"])
-                if i in keep:
-                    results[i] = _result(r)
-                b.remove(slot)
-                s.close()
+            if pending:
+                retire(pending.pop(0))
                 done += 1
-        elif nxt < n:
+            elif nxt < n and arrivals[order[nxt]] <= time.perf_counter() - t0:
+                if not admit():
+                    break
+                nxt += 1
+                k += 1
+            else:
+                break
+        if in_flight:
+            b.wait()
+            in_flight = False
+            pending += [slot for slot in live if slot not in pending and b.status(slot)[0] == native.SLOT_FINISHED]
+        if any(b.status(slot)[0] == native.SLOT_RUNNING for slot in live):
+            b.launch()
+            in_flight = True
+        elif not pending and nxt < n:
+            now = time.perf_counter() - t0
             time.sleep(max(0.0, min(0.01, arrivals[order[nxt]] - now)))
     gpu = b.info()["gpu_seconds"]
     b.close()
