@@ -52,6 +52,9 @@ struct thy_ctx {
     hipStream_t side = nullptr;          // general launch beside the deep-water launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // two replays in flight: start / end / control-copied events, pinned control copies
+    hipEvent_t rev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    Ctrl *h_snap = nullptr;
     int slot = 0;
     int cur = 0;                 // host mirror of ctrl.cur after a sync
     int tx = 64, ty = 16;
@@ -326,7 +329,9 @@ int build_order(thy_ctx *ctx, const T *D) {
     // per strip (segment sgi of strip row ty): cost class, and whether the deep-water
     // launch may take it: rows y0-4 .. y1+3 and its 64 columns clear of the grid edges
     // (no clamping, no first-order band), of the sponges, and of coast
-    const bool split = c->kernel == 0 && c->prec == 0;
+    // the deep-water path addresses each array through a buffer resource with a 32-bit
+    // byte range
+    const bool split = c->kernel == 0 && c->prec == 0 && (size_t)P * ny * sizeof(float) <= (size_t)INT32_MAX;
     const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
     const int n_strips = tiles_y * nseg;
     std::vector<int> scls(n_strips, 0);
@@ -364,12 +369,11 @@ int build_order(thy_ctx *ctx, const T *D) {
         const int t = ((ch << 3) + (bid & 7)) * chunk + (loc - ch * chunk);
         if (t < n_tiles) base.push_back(t);
     }
-    // strips in the same base order: the deep-water ones as they come (neighbours stay
-    // together in a workgroup), the general ones costliest first
-    // General strips are cut into strips of G rows (items in units of G rows).
+    // General strips in the base order, costliest first, cut into strips of G rows
+    // (items in units of G rows).
     const int G = (S > 8 && S % 8 == 0) ? 8 : S;
     c->gsrows = G;
-    std::vector<int> deep, gen, gcls;
+    std::vector<int> deep_rm, gen, gcls;
     c->bgen_h.clear();
     c->bgcls_h.clear();
     for (int t : base) {
@@ -378,10 +382,7 @@ int build_order(thy_ctx *ctx, const T *D) {
             const int sgi = tx * SW_WAVES + w;
             if (sgi >= nseg) continue;
             const int item = ty * nseg + sgi;
-            if (sdeep[item]) {
-                deep.push_back(item);
-                continue;
-            }
+            if (sdeep[item]) continue;
             c->bgen_h.push_back(item);
             c->bgcls_h.push_back(scls[item]);
             for (int sub = 0; sub < S / G && (ty * S + sub * G) < ny; sub++) {
@@ -403,16 +404,33 @@ int build_order(thy_ctx *ctx, const T *D) {
         c->bsub_h = gen;
         c->bsubcls_h.swap(c2);
     }
+    // Deep-water strips row-major, cut into 8 runs of (nearly) equal length, one per
+    // XCD: the workgroup at list slot b runs on XCD b mod 8, so each XCD sweeps one
+    // contiguous band and strips side by side share their halo columns in that XCD's
+    // L2.  Runs are padded to whole workgroups with -1 (idle wavefronts).
+    for (int item = 0; item < n_strips; item++)
+        if (sdeep[item]) deep_rm.push_back(item);
+    std::vector<int> deep;
+    if (!deep_rm.empty()) {
+        const int nd = (int)deep_rm.size(), q = nd / 8, rem = nd % 8;
+        const int wmax = (q + (rem > 0 ? 1 : 0) + SW_WAVES - 1) / SW_WAVES;
+        deep.assign((size_t)wmax * 8 * SW_WAVES, -1);
+        for (int x = 0; x < 8; x++) {
+            const int size = q + (x < rem ? 1 : 0), start = x * q + std::min(x, rem);
+            for (int i = 0; i < size; i++)
+                deep[((size_t)(i / SW_WAVES) * 8 + x) * SW_WAVES + i % SW_WAVES] = deep_rm[start + i];
+        }
+    }
     std::stable_sort(base.begin(), base.end(), [&](int a, int b) { return cls[a] > cls[b]; });
     c->order_h = base;
     c->cls_h = cls;
-    c->bdeep_h = deep;
+    c->bdeep_h = deep_rm;
     if (c->gexec) {  // a captured step sequence holds the old tables and grid sizes
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
         c->gk = 0;
     }
-    const int cap = std::max(n_strips, ((ny + G - 1) / G) * nseg);
+    const int cap = std::max({n_strips, ((ny + G - 1) / G) * nseg, (int)deep.size()});
     if (c->list_cap < cap) {
         void *ptrs[] = {c->order, c->deep, c->gen, c->fb_list};
         for (void *p : ptrs)
@@ -671,7 +689,13 @@ int build_graph(thy_ctx *ctx, int K) {
     return THY_OK;
 }
 
-// Advance until the device stop predicate holds; graph replays of K steps.
+// Advance until the device stop predicate holds; graph replays of K steps.  The
+// control block after each replay is copied to pinned memory behind it.  While the
+// last replay's time, steps and mean dt say that the run goes on for two more full
+// replays (before t_end, max_steps, the next fault and the next snapshot), the next
+// replay is queued before the one in flight is waited for, so that the device does not
+// idle between replays.  A queued replay that finds the run stopped idles through its
+// launches (the stop predicate holds in every one of them).
 int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
     int r;
     while (ctx->slot != 0) {  // align to the graph's slot sequence
@@ -679,25 +703,61 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, hipGetErrorString(e));
     }
     if ((r = build_graph(ctx, K)) != THY_OK) return r;
-    double total = 0.0;
-    for (;;) {
-        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
-        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    Ctrl last;
+    if ((r = read_ctrl(ctx, &last)) != THY_OK) return r;
+    double total = 0.0, dt_mean = 0.0;
+    int head = 0, queued = 0;
+    auto enqueue = [&]() -> hipError_t {
+        const int i = (head + queued) % 2;
+        hipError_t e;
+        if ((e = hipEventRecord(ctx->rev[i][0], ctx->stream)) != hipSuccess) return e;
+        if ((e = hipGraphLaunch(ctx->gexec, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ctx->rev[i][1], ctx->stream)) != hipSuccess) return e;
         ctx->slot = (ctx->slot + K) % 3;
-        Ctrl c;
-        if ((r = read_ctrl(ctx, &c)) != THY_OK) return r;
+        if ((e = hipMemcpyAsync(&ctx->h_snap[i], &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost,
+                                ctx->stream)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(ctx->rev[i][2], ctx->stream)) != hipSuccess) return e;
+        queued++;
+        return hipSuccess;
+    };
+    auto goes_on = [&](const Ctrl &c) {
+        if (!(dt_mean > 0.0)) return false;
+        const double t2 = c.t + 2.5 * K * dt_mean;  // two replays, with a margin for dt to grow
+        if (!(t2 < c.t_end) || t2 >= c.next_save) return false;
+        if (c.max_steps > 0 && c.steps + 2 * (int64_t)K > c.max_steps) return false;
+        if (c.k < c.nf && t2 * c.T >= ctx->ftimes[c.k]) return false;
+        return true;
+    };
+    auto collect = [&](Ctrl &c) -> hipError_t {
+        hipError_t e = hipEventSynchronize(ctx->rev[head][2]);
+        if (e != hipSuccess) return e;
+        c = ctx->h_snap[head];
         float f = 0.f;
-        (void)hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
+        (void)hipEventElapsedTime(&f, ctx->rev[head][0], ctx->rev[head][1]);
         total += f;
+        head ^= 1;
+        queued--;
+        return hipSuccess;
+    };
+    HIPCHK(enqueue());
+    for (;;) {
+        if (queued == 1 && goes_on(last)) HIPCHK(enqueue());
+        Ctrl c;
+        HIPCHK(collect(c));
         ctx->cur = c.cur;
+        if (c.steps > last.steps) dt_mean = (c.t - last.t) / (double)(c.steps - last.steps);
+        last = c;
         // After an executed step dt_valid is 0; it stays 1 only when a launch of this
         // replay found the stop predicate true (every later launch then idled).
         if (c.dt_valid) {
+            while (queued > 0) HIPCHK(collect(c));  // a queued replay idled: same control block
+            ctx->cur = c.cur;
             *out = c;
             if (ms) *ms = total;
             return THY_OK;
         }
+        if (queued == 0) HIPCHK(enqueue());
     }
 }
 
@@ -797,6 +857,12 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     HIPCHK(hipEventCreate(&ctx->ev1));
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    for (int i = 0; i < 2; i++) {
+        HIPCHK(hipEventCreate(&ctx->rev[i][0]));
+        HIPCHK(hipEventCreate(&ctx->rev[i][1]));
+        HIPCHK(hipEventCreateWithFlags(&ctx->rev[i][2], hipEventDisableTiming));
+    }
+    HIPCHK(Pool::get().host_alloc((void **)&ctx->h_snap, 2 * sizeof(Ctrl)));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
     for (int k = 0; k < 2; k++) {
         HIPCHK(dev_alloc(&ctx->e[k], b));
@@ -870,6 +936,10 @@ void thy_destroy(thy_ctx *ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 3; j++)
+            if (ctx->rev[i][j]) (void)hipEventDestroy(ctx->rev[i][j]);
+    Pool::get().host_free(ctx->h_snap);
     if (ctx->stream) Pool::get().put_streams(ctx->stream, ctx->side);
     delete ctx;
 }
