@@ -8,8 +8,9 @@ seamounts, coastline along the east and north edges, four 100x50 km subfaults wi
 region starts.  Prints one JSON line (rank 0, N = 1):
 
   value        Gcell-steps/s over exactly --steps steps after --warmup steps
-  roofline     dominant kernel (the step kernel): algorithmic bytes per launch / measured
-               launch time (HIP events on the library's stream) vs 8 TB/s
+  roofline     the step (row sweep: deep-water launch beside the general launch, then the
+               hand-back launch): algorithmic bytes per step / measured step time (HIP
+               events on the library's stream around each graph replay) vs 8 TB/s
   cpu_baseline FP64 restatement of the reference scheme (the oracle) timed on the
                host cores for a bounded sample of the same workload
 """
@@ -27,7 +28,9 @@ sys.path.insert(0, os.path.join(ROOT, "tsunami-hysea_amd"))
 METRIC = "Gcell-steps/s at 12M cells; batched scenarios/hr + p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
 B_ALG = {False: 36.0, True: 44.0}  # bytes per cell-step (arrival tracking off / on)
-KERNEL_NAME = {"sweep": "k_sweep", "tile": "k_step"}
+# the launches of one step, and the profile file stem of its measured HBM traffic
+KERNEL_NAME = {"sweep": "k_sweep_deep + k_sweep (general, hand-back)", "tile": "k_step"}
+TRAFFIC_FILE = {"sweep": "sweep_step_12M_traffic.json", "tile": "k_step_12M_traffic.json"}
 
 
 def log(*a):
@@ -239,10 +242,10 @@ def main():
     log(f"timed {args.steps} steps: wall {wall*1e3:.2f} ms, gpu {gpu_s*1e3:.2f} ms, {value:.1f} Gcell-steps/s")
 
     traffic = None
-    tp = os.path.join(ROOT, "profiles", f"{KERNEL_NAME[args.kernel]}_12M_traffic.json")
+    tp = os.path.join(ROOT, "profiles", TRAFFIC_FILE[args.kernel])
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_step")
 
     cpu = None
     if not args.no_cpu_baseline:
