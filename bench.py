@@ -186,6 +186,7 @@ def main():
     ap.add_argument("--sync-every", type=int, default=48, help="steps per captured graph replay")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps-single", type=int, default=2, help="oracle steps timed on one thread (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for counter scaling)")
@@ -247,6 +248,14 @@ def main():
         with open(tp) as f:
             traffic = json.load(f).get("bytes_per_step")
 
+    # regime of the cells at the end of the timed steps: wet, and wet with h <= hpos (the
+    # WAF branch, friction and the near-dry filter act there)
+    h, _, _, _ = solver.state()
+    wet = h > prob.eps_h
+    regime = {"wet_fraction": float(np.mean(wet)),
+              "wet_h_le_hpos_fraction": float(np.mean(wet & (h <= prob.hpos)))}
+    del h, wet
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -255,6 +264,10 @@ def main():
         cpu = {"value": gc, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {prob.nx}x{prob.ny} grid after Okada "
                          f"({sec:.1f} s), FP64 oracle, OpenMP"}
+        if args.cpu_steps_single > 0:
+            g1, s1 = cpu_baseline(prob, args.cpu_steps_single, 1)
+            cpu["single_thread"] = {"value": g1, "cores": 1,
+                                    "sample": f"{args.cpu_steps_single} steps of the same grid ({s1:.1f} s)"}
     solver.close()
     batched = batched_phase(args.batch, args.batch_sample, args.sync_every) if args.batch > 0 else None
     serve = (serving_phase(args.serve_rate, args.serve_seconds, args.sync_every, not args.no_serve_sequential)
@@ -268,7 +281,7 @@ def main():
         "config": {"workload": f"{prob.nx}x{prob.ny} lon/lat grid ({ncell/1e6:.1f}M cells, 2 arc-min), "
                                "4 Okada subfaults, 64 gauges, open borders",
                    "cells": ncell, "track_arrival": track, "steps_per_sync": args.sync_every,
-                   "step_kernel": args.kernel, "autotune": tuned,
+                   "step_kernel": args.kernel, "autotune": tuned, "cell_regime": regime,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
