@@ -228,8 +228,8 @@ def main():
     tuned = None
     if args.kernel == "sweep" and args.strip_rows == 0:
         rows, tms = solver.autotune(reps=5)
-        tuned = {"strip_rows": rows, "ms_per_step": tms}
-        log(f"autotune: {rows}-row strips, {tms:.4f} ms per step")
+        tuned = {"strip_rows": rows, "general_rows": solver.general_rows, "ms_per_step": tms}
+        log(f"autotune: {rows}-row strips ({solver.general_rows}-row general strips), {tms:.4f} ms per step")
     t0 = time.perf_counter()
     r = solver.run_more(args.steps)
     wall = time.perf_counter() - t0

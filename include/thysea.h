@@ -65,6 +65,9 @@ typedef struct thy_params {
                           coast and borders (default), 1 2D LDS tiles, 2 row sweep with the
                           general faces everywhere */
     int strip_rows;    /* row-sweep kernel: rows per strip, 0: default */
+    int general_rows;  /* row-sweep kernel: rows per general strip beside the deep-water
+                          launch (must divide strip_rows), 0: default (taller strips when
+                          the general strips are many, 8-row strips when they are few) */
 } thy_params;
 
 /* One Okada fault, fields as a parameter-file fault line (src/Problema.cxx:128-148). */
@@ -180,10 +183,11 @@ int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 /* Strip-height autotuner of the row-sweep kernel.  No reference counterpart (the
    reference has one fixed launch shape).  Times `reps` step launches (HIP events) from
-   the current state for strip heights 16, 24, 32, 40, 48 and 64 rows and keeps the
-   fastest; the state, control blocks and dt slots are left as they were.  Call after
-   thy_upload_state, before or between runs. */
-int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_per_step);
+   the current state for pairs of strip height and general strip height (16/8, 24/8,
+   24/12, 32/8, 32/16, 36/18, 40/20, 48/16) and keeps the fastest; the state, control
+   blocks and dt slots are left as they were.  Call after thy_upload_state, before or
+   between runs.  general_rows may be null. */
+int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, double *ms_per_step);
 
 /* Wall time (ms) of the last kernel launch group, for benchmarking. */
 int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);

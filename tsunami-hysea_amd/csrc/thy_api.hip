@@ -62,6 +62,8 @@ struct thy_ctx {
     int kernel = 0;              // 0 row sweep (deep-water + general split), 1 2D tiles, 2 row sweep general only
     int srows = 32;              // row sweep: rows per strip
     int gsrows = 32;             //   rows per general strip beside the deep-water launch
+    int grows_req = 0;           //   requested general strip height (0: the default rule)
+    int ncu = 256;               // compute units of the device
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
     int order_n = 0;
     std::vector<int> order_h, cls_h;  // host copies: every block costliest first, and the classes
@@ -292,6 +294,9 @@ int pick_tile(thy_ctx *c) {
     if (c->prm.strip_rows < 0 || c->prm.strip_rows > 65536)
         return fail(c, THY_ERR_INVALID, "strip_rows out of range: " + std::to_string(c->prm.strip_rows));
     if (c->prm.strip_rows > 0) c->srows = c->prm.strip_rows;
+    if (c->prm.general_rows < 0 || c->prm.general_rows > 65536)
+        return fail(c, THY_ERR_INVALID, "general_rows out of range: " + std::to_string(c->prm.general_rows));
+    c->grows_req = c->prm.general_rows;
     // THYSEA_CHUNK_ROWS overrides the number of tile rows dealt to one XCD at a time
     if (const char *v = std::getenv("THYSEA_CHUNK_ROWS")) {
         const int r = std::atoi(v);
@@ -307,6 +312,24 @@ int pick_tile(thy_ctx *c) {
 // of the launch.  Workgroups of equal cost class keep the chunked XCD order of the
 // kernel.  The estimate uses the still-water depth: a row counts as coastal for a
 // 64-column segment when any cell of it is shallower than 3 hpos (or dry).
+// Rows per general strip for strips of S rows: the requested height when it divides S.
+// Otherwise, when the general launch is bound by throughput (8-row strips would number
+// more than 8 per CU), the largest divisor of S between
+// 8 and min(20, S / 2), whose halos are a smaller share; when it is bound by the time of
+// one strip (smaller grids), the smallest divisor of S from 8 up.  S itself if none.
+int general_rows_of(int S, int req, size_t n_general, int ncu) {
+    if (req > 0 && S % req == 0) return req;
+    const bool throughput = n_general * (size_t)((S + 7) / 8) > (size_t)8 * std::max(1, ncu);
+    if (throughput) {
+        for (int g = std::min(20, S / 2); g >= 8; g--)
+            if (S % g == 0) return g;
+    } else {
+        for (int g = 8; g < S; g++)
+            if (S % g == 0) return g;
+    }
+    return S;
+}
+
 template <typename T>
 int build_order(thy_ctx *ctx, const T *D) {
     thy_ctx *c = ctx;
@@ -370,8 +393,12 @@ int build_order(thy_ctx *ctx, const T *D) {
         if (t < n_tiles) base.push_back(t);
     }
     // General strips in the base order, costliest first, cut into strips of G rows
-    // (items in units of G rows).
-    const int G = (S > 8 && S % 8 == 0) ? 8 : S;
+    // (items in units of G rows): short enough that the few slow coastal and border
+    // strips do not set the launch's critical path, long enough that their halo rows
+    // stay a modest share.
+    size_t n_general = 0;
+    for (int item = 0; item < n_strips; item++) n_general += sdeep[item] ? 0 : 1;
+    const int G = general_rows_of(S, c->grows_req, n_general, c->ncu);
     c->gsrows = G;
     std::vector<int> deep_rm, gen, gcls;
     c->bgen_h.clear();
@@ -853,6 +880,11 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     if (r != THY_OK) return r;
     HIPCHK(hipSetDevice(0));
     HIPCHK(Pool::get().get_streams(ctx->stream, ctx->side));
+    {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));
     HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
@@ -977,7 +1009,7 @@ int thy_initial_dt(thy_ctx *ctx, double *dt) {
     return ctx->prec ? initial_dt_t<double>(ctx, dt) : initial_dt_t<float>(ctx, dt);
 }
 
-int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
+int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, double *ms_best) {
     if (!ctx) return THY_ERR_INVALID;
     if (ctx->kernel == 1) return fail(ctx, THY_ERR_INVALID, "autotune covers the row-sweep kernel");
     if (ctx->order_h.empty()) return fail(ctx, THY_ERR_INVALID, "autotune needs an uploaded state");
@@ -997,12 +1029,15 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
     c.dt = dt;
     c.dt_valid = 1;
     c.cur = ctx->cur;
-    const int cand[] = {16, 24, 32, 40, 48, 64};
-    int best = ctx->srows;
+    // (strip rows, general strip rows)
+    const int cand[][2] = {{16, 8}, {24, 8}, {24, 12}, {32, 8}, {32, 16}, {36, 18}, {40, 20}, {48, 16}};
+    int best = ctx->srows, best_g = ctx->grows_req;
     double best_ms = 1e30;
-    for (int S : cand) {
-        if (S > ctx->ny && S != cand[0]) continue;
+    for (const auto &sg : cand) {
+        const int S = sg[0];
+        if (S > ctx->ny && S != cand[0][0]) continue;
         ctx->srows = S;
+        ctx->grows_req = sg[1];
         r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
         if (r != THY_OK) return r;
         std::vector<float> t;
@@ -1024,15 +1059,18 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, double *ms_best) {
         if (med < best_ms) {
             best_ms = med;
             best = S;
+            best_g = sg[1];
         }
     }
     ctx->srows = best;
+    ctx->grows_req = best_g;
     r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
     if (r != THY_OK) return r;
     ctx->slot = saved_slot;
     HIPCHK(hipMemcpy(ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
     if (strip_rows) *strip_rows = best;
+    if (general_rows) *general_rows = ctx->gsrows;
     if (ms_best) *ms_best = best_ms;
     return THY_OK;
 }

@@ -225,7 +225,7 @@ int main(int argc, char *argv[]) {
     if (precision == 0 && env_int("THYSEA_AUTOTUNE", 1)) {
         int rows = 0;
         double ms = 0.0;
-        if (thy_autotune(ctx, 3, &rows, &ms) != THY_OK)
+        if (thy_autotune(ctx, 3, &rows, nullptr, &ms) != THY_OK)
             std::cerr << "Warning: autotune failed: " << thy_last_error(ctx) << std::endl;
     }
     RunState rs;

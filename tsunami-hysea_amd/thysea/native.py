@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("dif_at", C.c_double), ("hmin", C.c_double), ("sea_level", C.c_double), ("L", C.c_double),
                 ("H", C.c_double), ("Q", C.c_double), ("T", C.c_double), ("precision", C.c_int),
                 ("track_arrival", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("kernel", C.c_int),
-                ("strip_rows", C.c_int)]
+                ("strip_rows", C.c_int), ("general_rows", C.c_int)]
 
 KERNELS = {"sweep": 0, "tile": 1, "sweep_general": 2}
 
@@ -97,7 +97,7 @@ def lib():
         L.thy_footprint.restype = C.c_size_t
         L.thy_last_timing.argtypes = [C.c_void_p, _dp]
         L.thy_copy_probe.argtypes = [C.c_size_t, C.c_int, _dp]
-        L.thy_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
+        L.thy_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
         L.thy_batch_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
         L.thy_batch_destroy.argtypes = [C.c_void_p]
         L.thy_batch_last_error.argtypes = [C.c_void_p]
@@ -237,11 +237,14 @@ class Solver:
         return v.value
 
     def autotune(self, reps=5):
-        """Pick the fastest strip height of the row-sweep kernel (thy_autotune);
-        returns (strip_rows, ms per step)."""
+        """Pick the fastest strip height and general strip height of the row-sweep
+        kernel (thy_autotune); returns (strip_rows, ms per step) and keeps the general
+        strip height in self.general_rows."""
         rows = C.c_int()
+        grows = C.c_int()
         ms = C.c_double()
-        self._chk(lib().thy_autotune(self.h, int(reps), C.byref(rows), C.byref(ms)), "autotune")
+        self._chk(lib().thy_autotune(self.h, int(reps), C.byref(rows), C.byref(grows), C.byref(ms)), "autotune")
+        self.general_rows = grows.value
         return rows.value, ms.value
 
     def run_more(self, n_steps):
