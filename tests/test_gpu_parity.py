@@ -480,3 +480,30 @@ def test_full_size_split_and_mass():
         s.close()
     for x, y in zip(*out):
         assert np.array_equal(x, y), float(np.max(np.abs(x - y)))
+
+
+def test_batch_many_general_strips_matches_alone():
+    """Twelve scenarios on the 1000x800 serving grid: enough general strips that the
+    batch runs them as whole strips (a run alone cuts them shorter); every sampled
+    scenario still ends in the state of its run alone, bit for bit."""
+    from thysea import scenario as S
+    nat = _native()
+    probs = S.batch_problems(12)
+    alone = {}
+    for i in (0, 5, 11):
+        s = nat.Solver(probs[i])
+        s.run(max_steps=30, steps_per_sync=6)
+        alone[i] = _fields(s)
+        s.close()
+    b = nat.Batch(capacity=16, steps_per_sync=6)
+    solvers = [nat.Solver(p) for p in probs]
+    slots = [b.add(s, max_steps=30) for s in solvers]
+    while b.advance() > 0:
+        pass
+    for i, ref in alone.items():
+        assert b.status(slots[i])[1]["steps"] == 30
+        for x, y in zip(_fields(solvers[i]), ref):
+            assert np.array_equal(x, y), (i, float(np.max(np.abs(x - y))))
+    b.close()
+    for s in solvers:
+        s.close()
