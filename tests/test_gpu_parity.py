@@ -328,11 +328,13 @@ def _split_pair(pr, steps):
     return out
 
 
-def test_deep_split_identical_to_general_faces():
+@pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, -1, -1, -1), (1, -1, 2, 2)],
+                         ids=["open", "walls", "periodic_x"])
+def test_deep_split_identical_to_general_faces(borders):
     """The deep-water launch (interior blocks far from coast and borders) and the
     general launch evaluate the deep faces with the same functions: the split run is
-    bit-identical to the general-faces-only run."""
-    pr = make_problem(nx=400, ny=300, seamounts=6, seed=21)
+    bit-identical to the general-faces-only run, with open, wall and periodic borders."""
+    pr = make_problem(nx=400, ny=300, seamounts=6, seed=21, borders=borders)
     a, b = _split_pair(pr, 30)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
