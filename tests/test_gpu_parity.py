@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C-ABI) against the FP64 oracle.
 
-Tolerances (north star): eta, qx, qy within relative L-inf 1e-4 of the peak |eta|
+Tolerances: eta, qx, qy within relative L-inf 1e-4 of the peak |eta|
 after the same step count with the oracle's dt replayed; relative mass drift 1e-6
 with walls; max-height and gauge series matching.
 """
