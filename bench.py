@@ -199,8 +199,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only", file=sys.stderr)
+        print(f"bench.py: the time-stepping path runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding across GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     from thysea import native, scenario
