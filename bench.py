@@ -37,6 +37,18 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """Model name of the host CPU (Linux), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(prob, steps, threads):
     """Oracle (FP64 restatement of the reference scheme) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -263,7 +275,8 @@ def main():
         gc, sec = cpu_baseline(prob, args.cpu_steps, threads)
         cpu = {"value": gc, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {prob.nx}x{prob.ny} grid after Okada "
-                         f"({sec:.1f} s), FP64 oracle, OpenMP"}
+                         f"({sec:.1f} s), FP64 oracle, OpenMP",
+               "cpu_model": cpu_model()}
         if args.cpu_steps_single > 0:
             g1, s1 = cpu_baseline(prob, args.cpu_steps_single, 1)
             cpu["single_thread"] = {"value": g1, "cores": 1,
