@@ -64,10 +64,10 @@ typedef struct thy_params {
     int kernel;        /* step kernel: 0 row sweep with a deep-water launch for blocks far from
                           coast and borders (default), 1 2D LDS tiles, 2 row sweep with the
                           general faces everywhere */
-    int strip_rows;    /* row-sweep kernel: rows per strip, 0: default */
+    int strip_rows;    /* row-sweep kernel: rows per strip, 0: default (24) */
     int general_rows;  /* row-sweep kernel: rows per general strip beside the deep-water
                           launch (must divide strip_rows), 0: default (taller strips when
-                          the general strips are many, 8-row strips when they are few) */
+                          the general strips are many, 4-row strips when they are few) */
 } thy_params;
 
 /* One Okada fault, fields as a parameter-file fault line (src/Problema.cxx:128-148). */
@@ -183,8 +183,8 @@ int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 /* Strip-height autotuner of the row-sweep kernel.  No reference counterpart (the
    reference has one fixed launch shape).  Times `reps` step launches (HIP events) from
-   the current state for pairs of strip height and general strip height (16/8, 24/8,
-   24/12, 32/8, 32/16, 36/18, 40/20, 48/16) and keeps the fastest; the state, control
+   the current state for pairs of strip height and general strip height (16/4, 16/8,
+   24/4, 24/8, 24/12, 32/8, 32/16, 36/18, 40/20, 48/16) and keeps the fastest; the state, control
    blocks and dt slots are left as they were.  Call after thy_upload_state, before or
    between runs.  general_rows may be null. */
 int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, double *ms_per_step);

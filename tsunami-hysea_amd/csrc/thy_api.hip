@@ -60,7 +60,7 @@ struct thy_ctx {
     int tx = 64, ty = 16;
     int chunk_rows = 1;          // tile (strip) rows per XCD chunk of the step kernel's order
     int kernel = 0;              // 0 row sweep (deep-water + general split), 1 2D tiles, 2 row sweep general only
-    int srows = 32;              // row sweep: rows per strip
+    int srows = 24;              // row sweep: rows per strip
     int gsrows = 32;             //   rows per general strip beside the deep-water launch
     int grows_req = 0;           //   requested general strip height (0: the default rule)
     int ncu = 256;               // compute units of the device
@@ -107,6 +107,32 @@ int fail(thy_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
 }
+
+// A context's copies and fills go through its own stream and are complete on return.
+// The null stream would order them against every blocking stream, one that another
+// thread is capturing into a graph included, and so invalidate that capture.
+hipError_t ctx_copy(thy_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+
+hipError_t ctx_fill(thy_ctx *c, void *dst, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+
+// Entry points run with the calling thread's stream-capture mode relaxed.  Their
+// allocations and synchronous copies touch only their own context's or batch's
+// streams, none of which another thread can be capturing; under the default (global)
+// mode the runtime would invalidate any capture in progress on another thread, such as
+// a batch's step sequence being recorded while a worker thread sets up the next context.
+struct RelaxedCapture {
+    hipStreamCaptureMode m = hipStreamCaptureModeRelaxed;
+    RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&m); }
+    ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&m); }
+    RelaxedCapture(const RelaxedCapture &) = delete;
+    RelaxedCapture &operator=(const RelaxedCapture &) = delete;
+};
 
 #define HIPCHK(call)                                                                   \
     do {                                                                               \
@@ -314,9 +340,9 @@ int pick_tile(thy_ctx *c) {
 // 64-column segment when any cell of it is shallower than 3 hpos (or dry).
 // Rows per general strip for strips of S rows: the requested height when it divides S.
 // Otherwise, when the general launch is bound by throughput (8-row strips would number
-// more than 8 per CU), the largest divisor of S between
-// 8 and min(20, S / 2), whose halos are a smaller share; when it is bound by the time of
-// one strip (smaller grids), the smallest divisor of S from 8 up.  S itself if none.
+// more than 8 per CU), the largest divisor of S between 8 and min(20, S / 2), whose
+// halos are a smaller share; when it is bound by the time of one strip (smaller grids),
+// the smallest divisor of S from 4 up, the shortest walk.  S itself if none.
 int general_rows_of(int S, int req, size_t n_general, int ncu) {
     if (req > 0 && S % req == 0) return req;
     const bool throughput = n_general * (size_t)((S + 7) / 8) > (size_t)8 * std::max(1, ncu);
@@ -324,7 +350,7 @@ int general_rows_of(int S, int req, size_t n_general, int ncu) {
         for (int g = std::min(20, S / 2); g >= 8; g--)
             if (S % g == 0) return g;
     } else {
-        for (int g = 8; g < S; g++)
+        for (int g = 4; g < S; g++)
             if (S % g == 0) return g;
     }
     return S;
@@ -470,13 +496,13 @@ int build_order(thy_ctx *ctx, const T *D) {
         c->list_cap = cap;
     }
     if (!c->fb_count) HIPCHK(dev_alloc(&c->fb_count, 3 * sizeof(unsigned int)));
-    HIPCHK(hipMemset(c->fb_count, 0, 3 * sizeof(unsigned int)));
+    HIPCHK(ctx_fill(ctx, c->fb_count, 3 * sizeof(unsigned int)));
     c->n_order = n_tiles;
     c->n_deep = (int)deep.size();
     c->n_gen = (int)gen.size();
-    HIPCHK(hipMemcpy(c->order, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!deep.empty()) HIPCHK(hipMemcpy(c->deep, deep.data(), deep.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!gen.empty()) HIPCHK(hipMemcpy(c->gen, gen.data(), gen.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, c->order, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!deep.empty()) HIPCHK(ctx_copy(ctx, c->deep, deep.data(), deep.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!gen.empty()) HIPCHK(ctx_copy(ctx, c->gen, gen.data(), gen.size() * sizeof(int), hipMemcpyHostToDevice));
     return THY_OK;
 }
 
@@ -484,7 +510,7 @@ int build_order(thy_ctx *ctx, const T *D) {
 template <typename T>
 int rebuild_order_t(thy_ctx *ctx) {
     std::vector<T> D((size_t)ctx->pitch * ctx->ny);
-    HIPCHK(hipMemcpy(D.data(), ctx->D, D.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, D.data(), ctx->D, D.size() * sizeof(T), hipMemcpyDeviceToHost));
     return build_order<T>(ctx, D.data());
 }
 
@@ -538,10 +564,10 @@ int download_t(thy_ctx *ctx, double *s1, double *s2) {
     const size_t b = (size_t)P * ny * sizeof(T);
     std::vector<T> e((size_t)P * ny), D((size_t)P * ny), qx((size_t)P * ny), qy((size_t)P * ny);
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(e.data(), ctx->e[ctx->cur], b, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(D.data(), ctx->D, b, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(qx.data(), ctx->qx[ctx->cur], b, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(qy.data(), ctx->qy[ctx->cur], b, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, e.data(), ctx->e[ctx->cur], b, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, D.data(), ctx->D, b, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, qx.data(), ctx->qx[ctx->cur], b, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, qy.data(), ctx->qy[ctx->cur], b, hipMemcpyDeviceToHost));
     for (int j = 0; j < ny; j++) {
         for (int i = 0; i < nx; i++) {
             const size_t c = (size_t)j * nx + i, o = (size_t)j * P + i;
@@ -560,19 +586,19 @@ int download_t(thy_ctx *ctx, double *s1, double *s2) {
 
 int read_ctrl(thy_ctx *ctx, Ctrl *c) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(c, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, c, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
     return THY_OK;
 }
 
 int write_ctrl(thy_ctx *ctx, const Ctrl &c) {
-    HIPCHK(hipMemcpy(&ctx->ctrl[ctx->slot], &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, &ctx->ctrl[ctx->slot], &c, sizeof(Ctrl), hipMemcpyHostToDevice));
     return THY_OK;
 }
 
 int reset_dt_slots(thy_ctx *ctx) {
     unsigned int v[3] = {0x7f7fffffu, 0x7f7fffffu, 0x7f7fffffu};
-    HIPCHK(hipMemcpy(ctx->dtmin, v, sizeof(v), hipMemcpyHostToDevice));
-    if (ctx->fb_count) HIPCHK(hipMemset(ctx->fb_count, 0, 3 * sizeof(unsigned int)));
+    HIPCHK(ctx_copy(ctx, ctx->dtmin, v, sizeof(v), hipMemcpyHostToDevice));
+    if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
     return THY_OK;
 }
 
@@ -631,11 +657,11 @@ int initial_dt_t(thy_ctx *ctx, double *dt) {
     const int64_t n = (int64_t)ctx->nx * ctx->ny;
     StepArgs<T> a = make_args<T>(ctx);
     unsigned int big = 0x7f7fffffu;
-    HIPCHK(hipMemcpy(&ctx->scratch_u[0], &big, sizeof(big), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, &ctx->scratch_u[0], &big, sizeof(big), hipMemcpyHostToDevice));
     hipLaunchKernelGGL((k_initial_dt<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, a, ctx->cur, &ctx->scratch_u[0]);
     HIPCHK(hipGetLastError());
     unsigned int r;
-    HIPCHK(hipMemcpy(&r, &ctx->scratch_u[0], sizeof(r), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, &r, &ctx->scratch_u[0], sizeof(r), hipMemcpyDeviceToHost));
     float f;
     memcpy(&f, &r, 4);
     *dt = (double)f;
@@ -678,7 +704,7 @@ int mass_t(thy_ctx *ctx, double *mass) {
                        ctx->mass_partial);
     HIPCHK(hipGetLastError());
     std::vector<double> part(kMassBlocks);
-    HIPCHK(hipMemcpy(part.data(), ctx->mass_partial, kMassBlocks * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, part.data(), ctx->mass_partial, kMassBlocks * sizeof(double), hipMemcpyDeviceToHost));
     double s = 0.0, comp = 0.0;
     for (double x : part) {
         double t = s + x;
@@ -811,6 +837,7 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ a
 extern "C" {
 
 int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy) {
+    RelaxedCapture relaxed;
     thy_ctx *ctx = nullptr;
     const int64_t n = (int64_t)(bytes / sizeof(float4));
     if (n <= 0 || reps <= 0) return THY_ERR_INVALID;
@@ -860,6 +887,7 @@ size_t thy_footprint(int nx, int ny, const thy_params *p) {
 }
 
 int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
+    RelaxedCapture relaxed;
     if (!g || !p || !out || g->nx < 2 || g->ny < 2) return THY_ERR_INVALID;
     thy_ctx *ctx = new (std::nothrow) thy_ctx();
     if (!ctx) return THY_ERR_HOST_MEMORY;
@@ -919,7 +947,7 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
         // mass weight per row: area*cos(phi), the invariant of the discrete scheme
         std::vector<double> w(ctx->ny);
         for (int j = 0; j < ctx->ny; j++) w[j] = g->area[j] * g->cosphi[j];
-        HIPCHK(hipMemcpy(ctx->area_d, w.data(), ctx->ny * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(ctx, ctx->area_d, w.data(), ctx->ny * sizeof(double), hipMemcpyHostToDevice));
     }
     if (ctx->prec) {
         std::vector<RowGeo<double>> rows;
@@ -927,32 +955,33 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
         fill_geometry(ctx, rows, fys);
         HIPCHK(dev_alloc(&ctx->row, rows.size() * sizeof(rows[0])));
         HIPCHK(dev_alloc(&ctx->fy, fys.size() * sizeof(fys[0])));
-        HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(ctx, ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(ctx, ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
     } else {
         std::vector<RowGeo<float>> rows;
         std::vector<FaceGeoY<float>> fys;
         fill_geometry(ctx, rows, fys);
         HIPCHK(dev_alloc(&ctx->row, rows.size() * sizeof(rows[0])));
         HIPCHK(dev_alloc(&ctx->fy, fys.size() * sizeof(fys[0])));
-        HIPCHK(hipMemcpy(ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(ctx, ctx->row, rows.data(), rows.size() * sizeof(rows[0]), hipMemcpyHostToDevice));
+        HIPCHK(ctx_copy(ctx, ctx->fy, fys.data(), fys.size() * sizeof(fys[0]), hipMemcpyHostToDevice));
     }
     // zero state so that padding columns hold finite values
     for (int k = 0; k < 2; k++) {
-        HIPCHK(hipMemset(ctx->e[k], 0, b));
-        HIPCHK(hipMemset(ctx->qx[k], 0, b));
-        HIPCHK(hipMemset(ctx->qy[k], 0, b));
+        HIPCHK(ctx_fill(ctx, ctx->e[k], b));
+        HIPCHK(ctx_fill(ctx, ctx->qx[k], b));
+        HIPCHK(ctx_fill(ctx, ctx->qy[k], b));
     }
-    HIPCHK(hipMemset(ctx->D, 0, b));
-    HIPCHK(hipMemset(ctx->emax, 0, b));
+    HIPCHK(ctx_fill(ctx, ctx->D, b));
+    HIPCHK(ctx_fill(ctx, ctx->emax, b));
     Ctrl c = idle_ctrl(ctx);
-    for (int s = 0; s < 3; s++) HIPCHK(hipMemcpy(&ctx->ctrl[s], &c, sizeof(c), hipMemcpyHostToDevice));
+    for (int s = 0; s < 3; s++) HIPCHK(ctx_copy(ctx, &ctx->ctrl[s], &c, sizeof(c), hipMemcpyHostToDevice));
     if ((r = reset_dt_slots(ctx)) != THY_OK) return r;
     return THY_OK;
 }
 
 void thy_destroy(thy_ctx *ctx) {
+    RelaxedCapture relaxed;
     if (!ctx) return;
     // no queued work may still use what goes back to the caches
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -979,37 +1008,44 @@ void thy_destroy(thy_ctx *ctx) {
 const char *thy_last_error(const thy_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int thy_upload_state(thy_ctx *ctx, const double *s1, const double *s2) {
+    RelaxedCapture relaxed;
     if (!ctx || !s1 || !s2) return THY_ERR_INVALID;
     return ctx->prec ? upload_t<double>(ctx, s1, s2) : upload_t<float>(ctx, s1, s2);
 }
 
 int thy_download_state(thy_ctx *ctx, double *s1, double *s2) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     return ctx->prec ? download_t<double>(ctx, s1, s2) : download_t<float>(ctx, s1, s2);
 }
 
 int thy_init_metrics(thy_ctx *ctx) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     return ctx->prec ? init_metrics_t<double>(ctx) : init_metrics_t<float>(ctx);
 }
 
 int thy_update_metrics(thy_ctx *ctx, double t) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     return ctx->prec ? update_metrics_t<double>(ctx, t) : update_metrics_t<float>(ctx, t);
 }
 
 int thy_apply_okada(thy_ctx *ctx, const thy_fault *f, int crop_flag, double crop_value, double *def_out) {
+    RelaxedCapture relaxed;
     if (!ctx || !f || crop_flag < 0 || crop_flag > 2) return THY_ERR_INVALID;
     return ctx->prec ? apply_okada_t<double>(ctx, f, crop_flag, crop_value, def_out)
                      : apply_okada_t<float>(ctx, f, crop_flag, crop_value, def_out);
 }
 
 int thy_initial_dt(thy_ctx *ctx, double *dt) {
+    RelaxedCapture relaxed;
     if (!ctx || !dt) return THY_ERR_INVALID;
     return ctx->prec ? initial_dt_t<double>(ctx, dt) : initial_dt_t<float>(ctx, dt);
 }
 
 int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, double *ms_best) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     if (ctx->kernel == 1) return fail(ctx, THY_ERR_INVALID, "autotune covers the row-sweep kernel");
     if (ctx->order_h.empty()) return fail(ctx, THY_ERR_INVALID, "autotune needs an uploaded state");
@@ -1019,8 +1055,8 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     Ctrl saved[3];
     unsigned int saved_dt[3];
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(saved, ctx->ctrl, sizeof(saved), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(saved_dt, ctx->dtmin, sizeof(saved_dt), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, saved, ctx->ctrl, sizeof(saved), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, saved_dt, ctx->dtmin, sizeof(saved_dt), hipMemcpyDeviceToHost));
     const int saved_slot = ctx->slot;
     double dt;
     if ((r = thy_initial_dt(ctx, &dt)) != THY_OK) return r;
@@ -1030,7 +1066,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     c.dt_valid = 1;
     c.cur = ctx->cur;
     // (strip rows, general strip rows)
-    const int cand[][2] = {{16, 8}, {24, 8}, {24, 12}, {32, 8}, {32, 16}, {36, 18}, {40, 20}, {48, 16}};
+    const int cand[][2] = {{16, 4}, {16, 8}, {24, 4}, {24, 8}, {24, 12}, {32, 8}, {32, 16}, {36, 18}, {40, 20}, {48, 16}};
     int best = ctx->srows, best_g = ctx->grows_req;
     double best_ms = 1e30;
     for (const auto &sg : cand) {
@@ -1044,7 +1080,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
         for (int k = 0; k <= reps; k++) {  // the first launch is a warm-up
             ctx->slot = 0;
             if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
-            if (ctx->fb_count) HIPCHK(hipMemset(ctx->fb_count, 0, 3 * sizeof(unsigned int)));
+            if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
             hipError_t e = launch_step(ctx);
             if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("autotune launch: ") + hipGetErrorString(e));
@@ -1067,8 +1103,8 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
     if (r != THY_OK) return r;
     ctx->slot = saved_slot;
-    HIPCHK(hipMemcpy(ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
     if (strip_rows) *strip_rows = best;
     if (general_rows) *general_rows = ctx->gsrows;
     if (ms_best) *ms_best = best_ms;
@@ -1076,6 +1112,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
 }
 
 int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     int r;
     Ctrl c = idle_ctrl(ctx);
@@ -1085,17 +1122,17 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
     if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
     unsigned int big = 0x7f7fffffu;
     const int s = ctx->slot;
-    HIPCHK(hipMemcpy(&ctx->dtmin[s], &big, sizeof(big), hipMemcpyHostToDevice));
-    if (ctx->fb_count) HIPCHK(hipMemset(&ctx->fb_count[s], 0, sizeof(unsigned int)));
+    HIPCHK(ctx_copy(ctx, &ctx->dtmin[s], &big, sizeof(big), hipMemcpyHostToDevice));
+    if (ctx->fb_count) HIPCHK(ctx_fill(ctx, &ctx->fb_count[s], sizeof(unsigned int)));
     hipError_t e = launch_step(ctx);
     if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     Ctrl o;
-    HIPCHK(hipMemcpy(&o, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, &o, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
     ctx->cur = o.cur;
     if (dt_next) {
         unsigned int v;
-        HIPCHK(hipMemcpy(&v, &ctx->dtmin[s], sizeof(v), hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(ctx, &v, &ctx->dtmin[s], sizeof(v), hipMemcpyDeviceToHost));
         float f;
         memcpy(&f, &v, 4);
         *dt_next = (double)f;
@@ -1104,11 +1141,13 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
 }
 
 int thy_get_field(thy_ctx *ctx, int field, float *out) {
+    RelaxedCapture relaxed;
     if (!ctx || !out || field < 0 || field > 5) return THY_ERR_INVALID;
     return ctx->prec ? get_field_t<double>(ctx, field, out) : get_field_t<float>(ctx, field, out);
 }
 
 int thy_mass(thy_ctx *ctx, double *mass) {
+    RelaxedCapture relaxed;
     if (!ctx || !mass) return THY_ERR_INVALID;
     return ctx->prec ? mass_t<double>(ctx, mass) : mass_t<float>(ctx, mass);
 }
@@ -1121,6 +1160,7 @@ int thy_last_timing(thy_ctx *ctx, double *ms) {
 
 int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v, float *eta_min, float *eta_max,
                    float *bathy) {
+    RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
     const int ns = ctx->nsamples, ng = ctx->ng;
     if (time) std::copy(ctx->g_time.begin(), ctx->g_time.begin() + ns, time);
@@ -1128,7 +1168,7 @@ int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v, fl
     if (u) std::copy(ctx->g_u.begin(), ctx->g_u.begin() + (size_t)ns * ng, u);
     if (v) std::copy(ctx->g_v.begin(), ctx->g_v.begin() + (size_t)ns * ng, v);
     std::vector<int> pos(ng);
-    if (ng) HIPCHK(hipMemcpy(pos.data(), ctx->gpos, ng * sizeof(int), hipMemcpyDeviceToHost));
+    if (ng) HIPCHK(ctx_copy(ctx, pos.data(), ctx->gpos, ng * sizeof(int), hipMemcpyDeviceToHost));
     if (eta_min || eta_max) {
         for (int g = 0; g < ng; g++) {
             float mn = 1e30f, mx = -1e30f;
@@ -1159,7 +1199,7 @@ static int drain_log(thy_ctx *ctx, int64_t upto) {
         return THY_OK;
     }
     std::vector<double> buf((size_t)ctx->logcap * 2);
-    HIPCHK(hipMemcpy(buf.data(), ctx->steplog, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, buf.data(), ctx->steplog, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t it = std::max(ctx->logged, upto - ctx->logcap); it < upto; it++) {
         const int64_t li = it % ctx->logcap;
         ctx->on_step(ctx->user, it + 1, buf[2 * li], buf[2 * li + 1]);
@@ -1223,8 +1263,8 @@ static int collect_gauges(thy_ctx *ctx) {
     ctx->g_v.assign((size_t)ns * ng, 0.f);
     if (ns > 0 && ng > 0) {
         std::vector<float> buf((size_t)ns * ng * 3);
-        HIPCHK(hipMemcpy(buf.data(), ctx->gbuf, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(ctx->g_time.data(), ctx->gtime, ns * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(ctx, buf.data(), ctx->gbuf, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(ctx, ctx->g_time.data(), ctx->gtime, ns * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < (size_t)ns * ng; i++) {
             ctx->g_eta[i] = buf[3 * i];
             ctx->g_u[i] = buf[3 * i + 1];
@@ -1257,7 +1297,7 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
     const std::vector<double> &ft = ctx->ftimes;
     if (ctx->ftime) { dev_free(ctx->ftime); ctx->ftime = nullptr; }
     HIPCHK(dev_alloc((void **)&ctx->ftime, ft.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx->ftime, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, ctx->ftime, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
     // gauges
     if (ctx->gpos) { dev_free(ctx->gpos); ctx->gpos = nullptr; }
     if (ctx->gbuf) { dev_free(ctx->gbuf); ctx->gbuf = nullptr; }
@@ -1270,7 +1310,7 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
         ctx->gcap = (int)std::min(nsd, 1e6);
     }
     HIPCHK(dev_alloc((void **)&ctx->gpos, std::max(1, ctx->ng) * sizeof(int)));
-    if (ctx->ng) HIPCHK(hipMemcpy(ctx->gpos, cfg->gauge_pos, ctx->ng * sizeof(int), hipMemcpyHostToDevice));
+    if (ctx->ng) HIPCHK(ctx_copy(ctx, ctx->gpos, cfg->gauge_pos, ctx->ng * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(dev_alloc((void **)&ctx->gbuf, std::max<size_t>(1, (size_t)ctx->gcap * ctx->ng * 3) * sizeof(float)));
     HIPCHK(dev_alloc((void **)&ctx->gtime, std::max(1, ctx->gcap) * sizeof(float)));
     if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; ctx->gk = 0; }
@@ -1319,6 +1359,7 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
 }
 
 int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
+    RelaxedCapture relaxed;
     if (!ctx || !cfg || !res) return THY_ERR_INVALID;
     int r;
     if ((r = prepare_run(ctx, cfg, res)) != THY_OK) return r;
@@ -1329,6 +1370,7 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res) {
 }
 
 int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res) {
+    RelaxedCapture relaxed;
     if (!ctx || !res || n_steps < 0) return THY_ERR_INVALID;
     if (!ctx->run_active) return fail(ctx, THY_ERR_INVALID, "thy_run_more without a previous thy_run");
     int r;
@@ -1529,6 +1571,7 @@ int batch_faults(thy_batch *b, thy_ctx *c, Ctrl &cb, bool &changed) {
 extern "C" {
 
 int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_batch **out) {
+    RelaxedCapture relaxed;
     if (!out || capacity < 1 || capacity > 65536) return THY_ERR_INVALID;
     thy_batch *b = new (std::nothrow) thy_batch();
     if (!b) return THY_ERR_HOST_MEMORY;
@@ -1550,11 +1593,11 @@ int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_ba
     BHIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     BHIPCHK(hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming));
     BHIPCHK(dev_alloc(&b->d_args, (size_t)capacity * sizeof(StepArgs<float>)));
-    BHIPCHK(hipMemset(b->d_args, 0, (size_t)capacity * sizeof(StepArgs<float>)));
+    BHIPCHK(hipMemsetAsync(b->d_args, 0, (size_t)capacity * sizeof(StepArgs<float>), b->stream));
     BHIPCHK(dev_alloc(&b->d_counts, 2 * sizeof(int)));
-    BHIPCHK(hipMemset(b->d_counts, 0, 2 * sizeof(int)));
+    BHIPCHK(hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), b->stream));
     BHIPCHK(dev_alloc(&b->d_fb_count, 3 * sizeof(unsigned int)));
-    BHIPCHK(hipMemset(b->d_fb_count, 0, 3 * sizeof(unsigned int)));
+    BHIPCHK(hipMemsetAsync(b->d_fb_count, 0, 3 * sizeof(unsigned int), b->stream));
     BHIPCHK(dev_alloc(&b->d_ctrl, (size_t)capacity * sizeof(Ctrl)));
     BHIPCHK(Pool::get().host_alloc((void **)&b->h_ctrl, (size_t)capacity * sizeof(Ctrl)));
     BHIPCHK(Pool::get().host_alloc((void **)&b->h_args, (size_t)capacity * sizeof(StepArgs<float>)));
@@ -1566,6 +1609,7 @@ int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_ba
 }
 
 void thy_batch_destroy(thy_batch *b) {
+    RelaxedCapture relaxed;
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->side) (void)hipStreamSynchronize(b->side);
@@ -1586,6 +1630,7 @@ void thy_batch_destroy(thy_batch *b) {
 const char *thy_batch_last_error(thy_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
 int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot_out) {
+    RelaxedCapture relaxed;
     if (!b || !ctx || !cfg || !slot_out) return THY_ERR_INVALID;
     if (ctx->prec != 0 || ctx->kernel == 1)
         return bfail(b, THY_ERR_INVALID, "batched scenarios run the FP32 row-sweep kernel only");
@@ -1622,6 +1667,7 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
 }
 
 int thy_batch_launch(thy_batch *b) {
+    RelaxedCapture relaxed;
     if (!b) return THY_ERR_INVALID;
     if (b->in_flight) return bfail(b, THY_ERR_INVALID, "a launch is in flight");
     int r;
@@ -1647,6 +1693,7 @@ int thy_batch_launch(thy_batch *b) {
 }
 
 int thy_batch_ready(thy_batch *b, int *done) {
+    RelaxedCapture relaxed;
     if (!b || !done) return THY_ERR_INVALID;
     if (!b->in_flight) {
         *done = 1;
@@ -1660,6 +1707,7 @@ int thy_batch_ready(thy_batch *b, int *done) {
 }
 
 int thy_batch_wait(thy_batch *b, int *n_running) {
+    RelaxedCapture relaxed;
     if (!b) return THY_ERR_INVALID;
     int r;
     if (b->in_flight) {
@@ -1703,6 +1751,7 @@ int thy_batch_wait(thy_batch *b, int *n_running) {
 }
 
 int thy_batch_advance(thy_batch *b, int *n_running) {
+    RelaxedCapture relaxed;
     int r = thy_batch_launch(b);
     if (r != THY_OK) return r;
     return thy_batch_wait(b, n_running);
@@ -1716,11 +1765,12 @@ int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res) {
 }
 
 int thy_batch_remove(thy_batch *b, int slot) {
+    RelaxedCapture relaxed;
     if (!b || slot < 0 || slot >= b->capacity || b->state[slot] == THY_SLOT_FREE) return THY_ERR_INVALID;
     if (b->state[slot] == THY_SLOT_RUNNING) {
         // a launch in flight may be stepping it: wait, and take it out of every later one
         BHIPCHK(hipStreamSynchronize(b->stream));
-        BHIPCHK(hipMemset(b->d_args + slot, 0, sizeof(StepArgs<float>)));
+        BHIPCHK(hipMemsetAsync(b->d_args + slot, 0, sizeof(StepArgs<float>), b->stream));
     }
     // a finished slot's argument block was cleared when it finished
     b->ctx[slot]->run_active = false;
@@ -1732,7 +1782,10 @@ int thy_batch_remove(thy_batch *b, int slot) {
     return THY_OK;
 }
 
-void thy_release_cached(void) { Pool::get().trim(); }
+void thy_release_cached(void) {
+    RelaxedCapture relaxed;
+    Pool::get().trim();
+}
 
 int thy_batch_info(thy_batch *b, int *running, size_t *bytes_used, double *gpu_seconds) {
     if (!b) return THY_ERR_INVALID;

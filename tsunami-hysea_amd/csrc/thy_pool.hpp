@@ -35,7 +35,8 @@ class Pool {
     hipError_t host_alloc(void **p, size_t bytes) { return alloc(host_, p, bytes); }
     void host_free(void *p) { give_back(host_, p); }
 
-    // a pair of streams (default flags and priority), cached or new
+    // a pair of streams (default priority), cached or new; non-blocking, so that work
+    // other code puts on the null stream does not order against them
     hipError_t get_streams(hipStream_t &a, hipStream_t &b) {
         {
             std::lock_guard<std::mutex> g(m_);
@@ -47,8 +48,8 @@ class Pool {
             }
         }
         a = b = nullptr;
-        hipError_t e = hipStreamCreate(&a);
-        if (e == hipSuccess) e = hipStreamCreate(&b);
+        hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
         if (e != hipSuccess) {
             if (a) (void)hipStreamDestroy(a);
             a = b = nullptr;

@@ -126,7 +126,7 @@ def release_cached():
     lib().thy_release_cached()
 
 
-def params_of(prob, precision=0, track_arrival=True, tile=(0, 0), kernel="sweep", strip_rows=0):
+def params_of(prob, precision=0, track_arrival=True, tile=(0, 0), kernel="sweep", strip_rows=0, general_rows=0):
     p = Params()
     p.borde_sup, p.borde_inf, p.borde_izq, p.borde_der = prob.borde_sup, prob.borde_inf, prob.borde_izq, prob.borde_der
     p.sponge_sup, p.sponge_inf = prob.sponge_sup, prob.sponge_inf
@@ -140,6 +140,7 @@ def params_of(prob, precision=0, track_arrival=True, tile=(0, 0), kernel="sweep"
     p.tile_x, p.tile_y = int(tile[0]), int(tile[1])
     p.kernel = KERNELS[kernel]
     p.strip_rows = int(strip_rows)
+    p.general_rows = int(general_rows)
     return p
 
 
@@ -154,12 +155,12 @@ class Solver:
     """One grid on the GPU, driven through the C-ABI."""
 
     def __init__(self, prob, precision=0, track_arrival=True, tile=(0, 0), upload=True, kernel="sweep",
-                 strip_rows=0):
+                 strip_rows=0, general_rows=0):
         self.prob = prob
         self._keep = [np.ascontiguousarray(a, np.float64) for a in
                       (prob.lon, prob.lat, prob.area, prob.cosphi, prob.ancho, prob.alto)]
         g = Grid(prob.nx, prob.ny, *[a.ctypes.data_as(_dp) for a in self._keep])
-        self.params = params_of(prob, precision, track_arrival, tile, kernel, strip_rows)
+        self.params = params_of(prob, precision, track_arrival, tile, kernel, strip_rows, general_rows)
         h = C.c_void_p()
         rc = lib().thy_create(C.byref(g), C.byref(self.params), C.byref(h))
         self.h = h

@@ -8,6 +8,7 @@ Time-to-result of a request = from its arrival to its max-height, arrival-time a
 gauge arrays in host memory (input generation happens before the clock starts, as
 reading input files would)."""
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -19,15 +20,18 @@ def _result(r):
             "steps": r["steps"]}
 
 
-def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8):
+def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8,
+                workers=4):
     """Serve ``problems`` (arrival times in seconds, default all at 0) through one
     Batch.  Returns per-request records and, for indices in ``keep``, the results.
 
-    The host work of a request (context set-up, upload, admission; read-out and
-    release once finished) runs while the GPU steps the batch: each step sequence is
-    launched asynchronously and the host works through finished and arrived requests
-    until it has completed (at most ``burst`` admissions while the GPU is idle, so
-    that stepping starts early)."""
+    The host work of a request runs while the GPU steps the batch: each step sequence
+    is launched asynchronously and the loop admits arrived requests and retires
+    finished ones until it has completed (at most ``burst`` admissions while the GPU is
+    idle, so that stepping starts early).  The work that touches one context only
+    (creating and uploading it; reading out its max height, arrival times and gauges
+    and releasing it) runs on ``workers`` threads; the batch itself is driven from
+    this thread."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
     order = list(np.argsort(arrivals, kind="stable"))
@@ -35,64 +39,114 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     rec = [dict(arrival=float(arrivals[i]), admitted=None, finished=None, steps=0) for i in range(n)]
     results = {}
     b = native.Batch(capacity, hbm_budget=hbm_budget, steps_per_sync=steps_per_sync)
-    live = {}      # slot -> (request index, solver)
-    pending = []   # finished slots not yet read out
+    pool = ThreadPoolExecutor(max_workers=max(1, workers))
+    live = {}        # slot -> (request index, solver)
+    pending = []     # finished slots not yet retired
+    prepared = {}    # request index -> future of its created and uploaded solver
+    readouts = []    # (request index, future of its results)
     t0 = time.perf_counter()
-    nxt = 0
-    done = 0
-    in_flight = False
+    state = {"nxt": 0, "nsub": 0, "done": 0}
 
-    def admit():
-        i = order[nxt]
-        s = native.Solver(problems[i])
+    def now():
+        return time.perf_counter() - t0
+
+    def prepare(i):
+        return native.Solver(problems[i])
+
+    def readout(s, n_samples, steps):
+        out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"), "steps": steps}
+        out.update(s.gauges(n_samples))
+        s.close()
+        return out, now()
+
+    def submit_prepares():
+        # keep up to two requests per worker in preparation, in arrival order
+        while (state["nsub"] < n and arrivals[order[state["nsub"]]] <= now()
+               and state["nsub"] - state["nxt"] < 2 * max(1, workers)):
+            i = order[state["nsub"]]
+            prepared[i] = pool.submit(prepare, i)
+            state["nsub"] += 1
+
+    def admit(block):
+        """Admit the next request: True, False (budget or slots full), None (not prepared yet)."""
+        i = order[state["nxt"]]
+        f = prepared.get(i)
+        if f is None or (not block and not f.done()):
+            return None
+        s = f.result()
         slot = b.add(s, steps_per_sync=steps_per_sync)
-        if slot is None:  # budget or slots full: wait for a finisher
-            s.close()
+        if slot is None:
             if not live:
+                s.close()
                 raise native.ThyError(f"request {i} does not fit the batch's HBM budget")
             return False
+        del prepared[i]
         live[slot] = (i, s)
-        rec[i]["admitted"] = time.perf_counter() - t0
+        rec[i]["admitted"] = now()
+        state["nxt"] += 1
         return True
 
     def retire(slot):
         i, s = live.pop(slot)
-        r = b.result(slot)
-        rec[i]["finished"] = time.perf_counter() - t0
-        rec[i]["steps"] = int(r["steps"])
-        if i in keep:
-            results[i] = _result(r)
+        r = b.status(slot)[1]
         b.remove(slot)
-        s.close()
+        readouts.append((i, pool.submit(readout, s, r["n_samples"], r["steps"])))
 
-    while done < n:
-        k = 0
-        while True:
-            if in_flight:
-                if b.ready():
-                    break
-            elif k >= burst:
-                break
-            if pending:
-                retire(pending.pop(0))
-                done += 1
-            elif nxt < n and arrivals[order[nxt]] <= time.perf_counter() - t0:
-                if not admit():
-                    break
-                nxt += 1
-                k += 1
+    def collect(block=False):
+        rest = []
+        for i, f in readouts:
+            if block or f.done():
+                out, t_done = f.result()
+                rec[i]["finished"] = t_done
+                rec[i]["steps"] = int(out["steps"])
+                if i in keep:
+                    results[i] = _result(out)
+                state["done"] += 1
             else:
+                rest.append((i, f))
+        readouts[:] = rest
+
+    in_flight = False
+    try:
+        while state["done"] < n:
+            submit_prepares()
+            k = 0
+            while True:
+                collect()
+                if in_flight:
+                    if b.ready():
+                        break
+                elif k >= burst:
+                    break
+                if pending:
+                    retire(pending.pop(0))
+                    continue
+                if state["nxt"] < n and arrivals[order[state["nxt"]]] <= now():
+                    ok = admit(block=not in_flight)
+                    if ok:
+                        k += 1
+                        submit_prepares()
+                        continue
+                    if ok is False:
+                        break
+                    submit_prepares()   # it arrived after the last submission
+                    time.sleep(0.0002)  # or it is still being prepared
+                    continue
                 break
-        if in_flight:
-            b.wait()
-            in_flight = False
-            pending += [slot for slot in live if slot not in pending and b.status(slot)[0] == native.SLOT_FINISHED]
-        if any(b.status(slot)[0] == native.SLOT_RUNNING for slot in live):
-            b.launch()
-            in_flight = True
-        elif not pending and nxt < n:
-            now = time.perf_counter() - t0
-            time.sleep(max(0.0, min(0.01, arrivals[order[nxt]] - now)))
+            if in_flight:
+                b.wait()
+                in_flight = False
+                pending += [slot for slot in live if slot not in pending and b.status(slot)[0] == native.SLOT_FINISHED]
+            if any(b.status(slot)[0] == native.SLOT_RUNNING for slot in live):
+                b.launch()
+                in_flight = True
+            elif not pending:
+                if readouts:
+                    collect(block=True)
+                elif state["nxt"] < n:
+                    time.sleep(max(0.0, min(0.01, arrivals[order[state["nxt"]]] - now())))
+    finally:
+        pool.shutdown(wait=True)
     gpu = b.info()["gpu_seconds"]
     b.close()
     return rec, results, gpu
