@@ -223,6 +223,47 @@ def test_serving_loop_matches_sequential():
         assert rec_b[i]["finished"] >= rec_b[i]["admitted"] >= rec_b[i]["arrival"]
 
 
+def test_threads_beside_batch_capture():
+    """Contexts set up, run (each capturing its own step sequences) and read out on
+    worker threads while the main thread records and replays batch step sequences:
+    no capture is invalidated, and every result equals a run alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    nat = _native()
+    side = [_scenario(96, 80, 20 + k, 0.4, 2.0, 200.0) for k in range(6)]
+    main = [_scenario(150, 64, 40 + k, 0.5, 3.0, 300.0) for k in range(3)]
+
+    def alone(pr):
+        s = nat.Solver(pr)
+        r = s.run(steps_per_sync=6)
+        out = (r["steps"], s.field("max_height"), r["g_eta"])
+        s.close()
+        return out
+
+    ref_side = [alone(pr) for pr in side]
+    ref_main = [alone(pr) for pr in main]
+    b = nat.Batch(capacity=8, steps_per_sync=6)
+    with ThreadPoolExecutor(max_workers=3) as pool:
+        futs = [pool.submit(alone, pr) for pr in side]
+        slots = {}
+        for k, pr in enumerate(main):
+            slots[b.add(nat.Solver(pr), steps_per_sync=6)] = k
+            b.advance()  # the set changed: the batch records a new step sequence
+        while b.advance() > 0:
+            pass
+        got_side = [f.result() for f in futs]
+    for slot, k in slots.items():
+        r = b.result(slot)
+        assert r["steps"] == ref_main[k][0]
+        assert np.array_equal(r["max_height"], ref_main[k][1]), k
+        assert np.array_equal(r["g_eta"], ref_main[k][2]), k
+        b.remove(slot).close()
+    b.close()
+    for k in range(len(side)):
+        assert got_side[k][0] == ref_side[k][0]
+        assert np.array_equal(got_side[k][1], ref_side[k][1]), k
+        assert np.array_equal(got_side[k][2], ref_side[k][2]), k
+
+
 def test_batch_budget_admission():
     """A scenario beyond the HBM budget is refused (None), and admitted once a
     finished one has been removed."""
