@@ -13,6 +13,11 @@
  *
  * Return codes follow the reference's int convention (0 ok, 1 device memory, 2 host
  * memory; src/GPU/ShallowWater.cu:275,285,319) extended with the codes below.
+ *
+ * Threads: calls on different contexts and batches may run on different threads at
+ * once, also while a batch records its step sequence (every entry point relaxes the
+ * calling thread's stream-capture mode for its duration).  One context, or one batch
+ * with the contexts it holds, is used from one thread at a time.
  */
 #ifndef THYSEA_H
 #define THYSEA_H
