@@ -236,6 +236,8 @@ int thy_batch_ready(thy_batch *b, int *done);
 int thy_batch_wait(thy_batch *b, int *n_running);
 /* thy_batch_launch, then thy_batch_wait. */
 int thy_batch_advance(thy_batch *b, int *n_running);
+/* State of a slot (THY_SLOT_*) and its run result; while it runs, steps and t_final
+   are those after the last completed launch. */
 int thy_batch_status(thy_batch *b, int slot, int *state, thy_run_result *res);
 /* Frees a slot (the context is not destroyed; remove a running scenario's slot before
    destroying its context).  Removing a running slot waits for a launch in flight. */

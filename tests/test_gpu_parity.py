@@ -186,6 +186,8 @@ def test_batch_matches_alone():
     solvers.append(nat.Solver(probs[2]))
     slots.append(b.add(solvers[2]))  # admitted while the launch is in flight
     assert b.wait() == 3
+    # progress of running slots: the first two stepped one sequence, the third none
+    assert [b.status(k)[1]["steps"] for k in slots] == [6, 6, 0]
     for _ in range(10000):
         if b.advance() == 0:
             break

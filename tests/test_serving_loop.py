@@ -24,11 +24,15 @@ class FakeError(RuntimeError):
 
 
 class FakeSolver:
-    def __init__(self, prob):
+    made = []
+
+    def __init__(self, prob, strip_rows=0, general_rows=0):
         time.sleep(0.001)  # set-up and upload take a while
         self.prob = prob
+        self.rows = strip_rows
         self.steps = 0
         self.closed = False
+        FakeSolver.made.append(self)
 
     def field(self, name):
         off = 0.0 if name == "max_height" else 0.5
@@ -59,6 +63,7 @@ class FakeBatch:
         self.polls = 0
         self.thread = threading.get_ident()
         self.admitted = []
+        self.gpu = 0.0
 
     def _main(self):
         assert threading.get_ident() == self.thread, "the batch is driven from one thread"
@@ -94,7 +99,9 @@ class FakeBatch:
         self._main()
         for k in self.flight:
             s = self.slots[k][0]
+            n0 = s.steps
             s.steps = min(s.prob["steps"], s.steps + self.K)
+            self.gpu += (s.steps - n0) * (abs(s.rows - 40) + 1) * 1e-6  # 40-row strips are the fastest
             if s.steps == s.prob["steps"]:
                 self.slots[k][1] = FINISHED
         self.flight = None
@@ -111,8 +118,13 @@ class FakeBatch:
         self.used -= s.prob["bytes"]
         return s
 
+    def advance(self):
+        self.launch()
+        self.wait()
+        return sum(v[1] == RUNNING for v in self.slots.values())
+
     def info(self):
-        return {"gpu_seconds": 0.0}
+        return {"gpu_seconds": self.gpu}
 
     def close(self):
         self._main()
@@ -188,3 +200,4 @@ def test_sequential_loop_returns_the_same_results(fake):
         assert rec_s[i]["steps"] == rec_b[i]["steps"]
         for k in ("max_height", "arrival_times", "g_eta"):
             assert np.array_equal(res_s[i][k], res_b[i][k])
+

@@ -1721,6 +1721,8 @@ int thy_batch_wait(thy_batch *b, int *n_running) {
             thy_ctx *c = b->ctx[i];
             Ctrl cb = b->h_ctrl[i];
             c->cur = cb.cur;
+            b->res[i].steps = cb.steps;  // progress, final once it has finished
+            b->res[i].t_final = cb.t;
             if (!cb.dt_valid) continue;  // stepped through the whole sequence
             // stopped: finished, or a fault is due (the only other stop a batched run has)
             if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {

@@ -21,7 +21,7 @@ def _result(r):
 
 
 def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8,
-                workers=4):
+                workers=4, solver_kw=None):
     """Serve ``problems`` (arrival times in seconds, default all at 0) through one
     Batch.  Returns per-request records and, for indices in ``keep``, the results.
 
@@ -31,7 +31,7 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     idle, so that stepping starts early).  The work that touches one context only
     (creating and uploading it; reading out its max height, arrival times and gauges
     and releasing it) runs on ``workers`` threads; the batch itself is driven from
-    this thread."""
+    this thread.  ``solver_kw`` goes to every native.Solver (strip heights, for one)."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
     order = list(np.argsort(arrivals, kind="stable"))
@@ -51,7 +51,7 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
         return time.perf_counter() - t0
 
     def prepare(i):
-        return native.Solver(problems[i])
+        return native.Solver(problems[i], **(solver_kw or {}))
 
     def readout(s, n_samples, steps):
         out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"), "steps": steps}
@@ -152,7 +152,7 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     return rec, results, gpu
 
 
-def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None):
+def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None, solver_kw=None):
     """The same requests one at a time through thy_run (first come, first served)."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
@@ -163,7 +163,7 @@ def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None):
         now = time.perf_counter() - t0
         if arrivals[i] > now:
             time.sleep(arrivals[i] - now)
-        s = native.Solver(problems[i])
+        s = native.Solver(problems[i], **(solver_kw or {}))
         start = time.perf_counter() - t0
         r = s.run(steps_per_sync=steps_per_sync)
         out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"),
