@@ -74,8 +74,14 @@ def batched_phase(n, sample, sync):
     probs = scenario.batch_problems(n)
     keep = list(range(min(sample, n)))
     log(f"batched: {n} scenarios on {probs[0].nx}x{probs[0].ny}")
+    # the batch's strip height for this grid, chosen before the clock starts (the plain
+    # runs keep the default, which is tuned for one scenario alone)
+    rows_b, cost_b = serving.tune_batch(probs, steps_per_sync=sync)
+    log("batched: strip heights " + ", ".join(f"{k}: {v:.2f}" for k, v in cost_b.items())
+        + f" us per scenario-step; {rows_b} rows")
     t0 = time.perf_counter()
-    rec_b, res_b, gpu_b = serving.run_batched(probs, steps_per_sync=sync, keep=keep)
+    rec_b, res_b, gpu_b = serving.run_batched(probs, steps_per_sync=sync, keep=keep,
+                                              solver_kw=dict(strip_rows=rows_b))
     wall_b = time.perf_counter() - t0
     log(f"batched: {wall_b:.2f} s wall, {gpu_b:.2f} s in step launches")
     t0 = time.perf_counter()
@@ -99,7 +105,8 @@ def batched_phase(n, sample, sync):
             "speedup_vs_sequential": (n / wall_b) / (1.0 / per_s),
             "gcell_steps_per_s": cells * steps / wall_b / 1e9,
             "p50_s": st["p50_s"], "p99_s": st["p99_s"], "wall_s": wall_b,
-            "sequential_sample": len(keep), "max_height_rel_err_vs_plain": err}
+            "sequential_sample": len(keep), "max_height_rel_err_vs_plain": err, "strip_rows": rows_b,
+            "strip_rows_tuning_us_per_scenario_step": {str(k): round(v, 3) for k, v in cost_b.items()}}
 
 
 def serving_phase(rate, horizon, sync, seq):

@@ -266,6 +266,16 @@ def test_threads_beside_batch_capture():
         assert np.array_equal(got_side[k][2], ref_side[k][2]), k
 
 
+def test_batch_tuner_times_every_candidate():
+    """tune_batch advances a batch at each candidate strip height and returns the
+    cheapest, with a positive cost for every candidate."""
+    from thysea import serving
+    probs = [_scenario(150, 96, 50 + k, 0.5, 2.0, 300.0) for k in range(4)]
+    rows, cost = serving.tune_batch(probs, candidates=(8, 16), n=4, launches=1, steps_per_sync=6)
+    assert sorted(cost) == [8, 16] and rows in cost
+    assert all(v > 0 for v in cost.values())
+
+
 def test_batch_budget_admission():
     """A scenario beyond the HBM budget is refused (None), and admitted once a
     finished one has been removed."""

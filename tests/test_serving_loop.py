@@ -201,3 +201,11 @@ def test_sequential_loop_returns_the_same_results(fake):
         for k in ("max_height", "arrival_times", "g_eta"):
             assert np.array_equal(res_s[i][k], res_b[i][k])
 
+
+def test_batch_tuner_picks_the_cheapest_strip_height(fake):
+    FakeSolver.made.clear()
+    probs = [dict(id=i, steps=1000, bytes=1) for i in range(40)]
+    rows, cost = serving.tune_batch(probs, candidates=(24, 32, 40, 48), n=8, launches=2, steps_per_sync=12)
+    assert rows == 40 and sorted(cost) == [24, 32, 40, 48]
+    assert cost[40] == pytest.approx(1.0) and cost[24] == pytest.approx(17.0)
+    assert len(FakeSolver.made) == 32 and all(s.closed for s in FakeSolver.made)

@@ -1585,8 +1585,10 @@ int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_ba
     b->state.assign(capacity, THY_SLOT_FREE);
     b->res.assign(capacity, thy_run_result{});
     b->launched.assign(capacity, 0);
-    BHIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    BHIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    // cached streams: creating and destroying streams with every batch would shift how
+    // the runtime spreads streams over the hardware queues, and the two launches of a
+    // step only overlap when the pair sits on two different queues
+    BHIPCHK(Pool::get().get_streams(b->stream, b->side));
     BHIPCHK(hipEventCreate(&b->ev0));
     BHIPCHK(hipEventCreate(&b->ev1));
     BHIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
@@ -1622,8 +1624,7 @@ void thy_batch_destroy(thy_batch *b) {
     hipEvent_t evs[] = {b->ev0, b->ev1, b->ev_fork, b->ev_join, b->ev_done};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
-    if (b->side) (void)hipStreamDestroy(b->side);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream) Pool::get().put_streams(b->stream, b->side);
     delete b;
 }
 

@@ -152,6 +152,43 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     return rec, results, gpu
 
 
+def tune_batch(problems, candidates=(24, 32, 40, 48), n=32, launches=2, steps_per_sync=48):
+    """Pick the strip height for batched runs of scenarios like ``problems``: for each
+    candidate, a batch of the first ``n`` of them is advanced by one step sequence
+    (capture, faults at t = 0), then by ``launches`` more that are timed.  Returns the
+    strip height with the least GPU time per scenario-step and {candidate: us per
+    scenario-step}.  Many scenarios side by side keep every CU busy, so the batch
+    favours taller strips (fewer halo rows) than one scenario alone, whose short
+    strips shorten the step's critical path; thy_autotune's choice for one scenario is
+    therefore not the batch's."""
+    cost = {}
+    for rows in candidates:
+        b = native.Batch(n, steps_per_sync=steps_per_sync)
+        solvers = [native.Solver(p, strip_rows=rows) for p in problems[:n]]
+        try:
+            slots = [b.add(s, steps_per_sync=steps_per_sync) for s in solvers]
+            if any(k is None for k in slots):
+                raise native.ThyError("the tuning batch does not fit the HBM budget")
+            b.advance()
+
+            def totals():
+                return b.info()["gpu_seconds"], sum(b.status(k)[1]["steps"] for k in slots)
+
+            g0, n0 = totals()
+            for _ in range(launches):
+                b.advance()
+            g1, n1 = totals()
+            if n1 > n0:
+                cost[rows] = (g1 - g0) / (n1 - n0) * 1e6
+        finally:
+            b.close()
+            for s in solvers:
+                s.close()
+    if not cost:
+        raise native.ThyError("no candidate strip height advanced a scenario")
+    return min(cost, key=cost.get), cost
+
+
 def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None, solver_kw=None):
     """The same requests one at a time through thy_run (first come, first served)."""
     n = len(problems)
