@@ -223,8 +223,15 @@ int thy_batch_create(int capacity, size_t hbm_budget, int steps_per_sync, thy_ba
 void thy_batch_destroy(thy_batch *b);
 const char *thy_batch_last_error(thy_batch *b);
 /* Prepares the run (metrics, initial dt, faults due at t = 0) and admits the context;
-   THY_ERR_BUDGET when no slot is free or its footprint would exceed the budget. */
+   THY_ERR_BUDGET when no slot is free or its footprint would exceed the budget (the
+   context is then left as it was).  cfg NULL: the context was prepared by
+   thy_batch_prepare, and admission only takes a slot. */
 int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot);
+/* The context-local part of thy_batch_add (run configuration, metrics, initial dt,
+   faults due at t = 0), for a thread other than the one that drives the batch: the
+   set-up then overlaps the batch's launches instead of holding up its driver.  A later
+   upload or thy_run voids it.  No counterpart in the reference (batched serving). */
+int thy_batch_prepare(thy_ctx *ctx, const thy_run_config *cfg);
 /* Queues one captured sequence of steps_per_sync steps for every running scenario and
    returns; THY_ERR_INVALID while a launch is in flight.  Does nothing when no
    scenario is running. */

@@ -266,6 +266,45 @@ def test_threads_beside_batch_capture():
         assert np.array_equal(got_side[k][2], ref_side[k][2]), k
 
 
+def test_batch_prepared_on_another_thread_matches_alone():
+    """Runs prepared by thy_batch_prepare on a worker thread (faults at t = 0 applied
+    there) and admitted with no run configuration give the results of runs alone; a
+    context admitted with no configuration and no preparation is refused."""
+    from concurrent.futures import ThreadPoolExecutor
+    nat = _native()
+    probs = [_scenario(96, 80, 61, 0.4, 2.0, 300.0), _scenario(150, 64, 62, 0.5, 3.0, 400.0)]
+    alone = []
+    for pr in probs:
+        s = nat.Solver(pr)
+        r = s.run(steps_per_sync=6)
+        alone.append((r["steps"], s.field("max_height"), s.field("arrival_times"), r["g_eta"]))
+        s.close()
+
+    def prepared(pr):
+        s = nat.Solver(pr)
+        s.prepare_batch(steps_per_sync=6)
+        return s
+
+    b = nat.Batch(capacity=4, steps_per_sync=6)
+    raw = nat.Solver(probs[0])
+    rc = nat.lib().thy_batch_add(b.h, raw.h, None, nat.C.byref(nat.C.c_int()))
+    assert rc == nat.STATUS_CODES["INVALID"]
+    raw.close()
+    with ThreadPoolExecutor(max_workers=2) as pool:
+        solvers = list(pool.map(prepared, probs))
+    slots = [b.add(s) for s in solvers]
+    while b.advance() > 0:
+        pass
+    for slot, ra in zip(slots, alone):
+        rb = b.result(slot)
+        assert rb["steps"] == ra[0]
+        assert np.array_equal(rb["max_height"], ra[1])
+        assert np.array_equal(rb["arrival_times"], ra[2])
+        assert np.array_equal(rb["g_eta"], ra[3])
+        b.remove(slot).close()
+    b.close()
+
+
 def test_batch_tuner_times_every_candidate():
     """tune_batch advances a batch at each candidate strip height and returns the
     cheapest, with a positive cost for every candidate."""

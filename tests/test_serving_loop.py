@@ -25,6 +25,7 @@ class FakeError(RuntimeError):
 
 class FakeSolver:
     made = []
+    prepared_on = []
 
     def __init__(self, prob, strip_rows=0, general_rows=0):
         time.sleep(0.001)  # set-up and upload take a while
@@ -32,7 +33,12 @@ class FakeSolver:
         self.rows = strip_rows
         self.steps = 0
         self.closed = False
+        self.batch_prepared = False
         FakeSolver.made.append(self)
+
+    def prepare_batch(self, steps_per_sync=0, max_steps=0):
+        FakeSolver.prepared_on.append(threading.get_ident())
+        self.batch_prepared = True
 
     def field(self, name):
         off = 0.0 if name == "max_height" else 0.5
@@ -63,6 +69,7 @@ class FakeBatch:
         self.polls = 0
         self.thread = threading.get_ident()
         self.admitted = []
+        self.unprepared = 0
         self.gpu = 0.0
 
     def _main(self):
@@ -75,6 +82,8 @@ class FakeBatch:
             return None
         slot = min(set(range(self.cap)) - set(self.slots))
         self.slots[slot] = [s, RUNNING]
+        self.unprepared += 0 if s.batch_prepared else 1
+        s.batch_prepared = False
         self.used += s.prob["bytes"]
         self.max_used = max(self.max_used, self.used)
         self.max_live = max(self.max_live, len(self.slots))
@@ -161,12 +170,16 @@ def _check_results(probs, rec, res):
 
 @pytest.mark.parametrize("workers", [1, 4])
 def test_every_request_finishes_with_its_own_results(fake, workers):
+    FakeSolver.prepared_on.clear()
     probs = _problems(40)
     rec, res, _ = serving.run_batched(probs, capacity=6, steps_per_sync=48, keep=range(40), workers=workers)
     _check_results(probs, rec, res)
     b = fake[0]
     assert b.max_live <= 6 and not b.slots
     assert b.admitted == list(range(40))  # admitted in arrival order
+    # runs are prepared on the worker threads, never on the thread that drives the batch
+    assert len(FakeSolver.prepared_on) == 40 and b.thread not in FakeSolver.prepared_on
+    assert b.unprepared == 0
 
 
 def test_budget_bounds_the_resident_scenarios(fake):

@@ -93,6 +93,8 @@ struct thy_ctx {
     thy_step_fn on_step = nullptr;
     void *user = nullptr;
     bool run_active = false;
+    bool batch_ready = false;    // prepared by thy_batch_prepare, not yet admitted
+    thy_run_result prep_res{};   // its run result so far (initial dt)
     double *steplog = nullptr;
     int logcap = 4096;
     int64_t logged = 0;  // iterations already delivered to on_step
@@ -519,6 +521,7 @@ int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     const int nx = ctx->nx, ny = ctx->ny, P = ctx->pitch;
     const double hmin = ctx->prm.hmin;
     const size_t n = (size_t)P * ny, b = n * sizeof(T);
+    ctx->batch_ready = false;  // a new state: a run prepared for it is void
     // e, D, qx, qy staged in pinned host memory (a cached block), copied asynchronously
     T *st = nullptr;
     HIPCHK(Pool::get().host_alloc((void **)&st, 4 * b));
@@ -1281,6 +1284,7 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
         return fail(ctx, THY_ERR_INVALID, "bad run configuration");
     int r;
     memset(res, 0, sizeof(*res));
+    ctx->batch_ready = false;  // a run prepared before is replaced
     ctx->K = cfg->steps_per_sync > 0 ? ((cfg->steps_per_sync + 2) / 3) * 3 : 24;
     ctx->faults.assign(cfg->faults, cfg->faults + cfg->n_faults);
     ctx->ftimes.assign(std::max(1, cfg->n_faults), 0.0);
@@ -1549,21 +1553,42 @@ int batch_graph(thy_batch *b) {
 }
 
 // apply every fault of a scenario whose time has come; Ctrl c is its control block
-int batch_faults(thy_batch *b, thy_ctx *c, Ctrl &cb, bool &changed) {
+// faults that came due at the control block's time: Okada, then the control block
+// rewritten into launch slot 0; errors are the context's
+int ctx_faults(thy_ctx *ctx, Ctrl &cb, bool &changed) {
     changed = false;
-    while (cb.k < cb.nf && cb.t * cb.T >= c->ftimes[cb.k]) {
-        const int r = thy_apply_okada(c, &c->faults[cb.k], c->crop_flag, c->crop_value, nullptr);
-        if (r != THY_OK) return bfail(b, r, std::string("okada: ") + thy_last_error(c));
+    while (cb.k < cb.nf && cb.t * cb.T >= ctx->ftimes[cb.k]) {
+        const int r = thy_apply_okada(ctx, &ctx->faults[cb.k], ctx->crop_flag, ctx->crop_value, nullptr);
+        if (r != THY_OK) return fail(ctx, r, std::string("okada: ") + thy_last_error(ctx));
         cb.k++;
         changed = true;
     }
     if (changed) {
-        BHIPCHK(hipStreamSynchronize(c->stream));
-        c->slot = 0;
-        const int r = write_ctrl(c, cb);
-        if (r != THY_OK) return bfail(b, r, thy_last_error(c));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->slot = 0;
+        return write_ctrl(ctx, cb);
     }
     return THY_OK;
+}
+
+int batch_faults(thy_batch *b, thy_ctx *c, Ctrl &cb, bool &changed) {
+    const int r = ctx_faults(c, cb, changed);
+    return r == THY_OK ? r : bfail(b, r, thy_last_error(c));
+}
+
+// the context-local part of admission: run configuration, metrics, initial dt, faults
+// due at t = 0 (src/GPU/ShallowWater.cu:305-391)
+int prepare_for_batch(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result &res) {
+    if (ctx->prec != 0 || ctx->kernel == 1)
+        return fail(ctx, THY_ERR_INVALID, "batched scenarios run the FP32 row-sweep kernel only");
+    if (cfg->save_interval >= 0.0) return fail(ctx, THY_ERR_INVALID, "batched scenarios take no snapshots");
+    if (ctx->order_h.empty()) return fail(ctx, THY_ERR_INVALID, "scenario state not uploaded");
+    int r;
+    if ((r = prepare_run(ctx, cfg, &res)) != THY_OK) return r;
+    Ctrl cb;
+    if ((r = read_ctrl(ctx, &cb)) != THY_OK) return r;
+    bool changed;
+    return ctx_faults(ctx, cb, changed);
 }
 
 }  // namespace
@@ -1630,13 +1655,19 @@ void thy_batch_destroy(thy_batch *b) {
 
 const char *thy_batch_last_error(thy_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
+int thy_batch_prepare(thy_ctx *ctx, const thy_run_config *cfg) {
+    RelaxedCapture relaxed;
+    if (!ctx || !cfg) return THY_ERR_INVALID;
+    ctx->batch_ready = false;
+    const int r = prepare_for_batch(ctx, cfg, ctx->prep_res);
+    if (r == THY_OK) ctx->batch_ready = true;
+    return r;
+}
+
 int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *slot_out) {
     RelaxedCapture relaxed;
-    if (!b || !ctx || !cfg || !slot_out) return THY_ERR_INVALID;
-    if (ctx->prec != 0 || ctx->kernel == 1)
-        return bfail(b, THY_ERR_INVALID, "batched scenarios run the FP32 row-sweep kernel only");
-    if (cfg->save_interval >= 0.0) return bfail(b, THY_ERR_INVALID, "batched scenarios take no snapshots");
-    if (ctx->order_h.empty()) return bfail(b, THY_ERR_INVALID, "scenario state not uploaded");
+    if (!b || !ctx || !slot_out) return THY_ERR_INVALID;
+    if (!cfg && !ctx->batch_ready) return bfail(b, THY_ERR_INVALID, "context not prepared (thy_batch_prepare)");
     for (int i = 0; i < b->capacity; i++)
         if (b->ctx[i] == ctx && b->state[i] != THY_SLOT_FREE) return bfail(b, THY_ERR_INVALID, "context already admitted");
     int slot = -1;
@@ -1645,13 +1676,14 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
     if (slot < 0) return bfail(b, THY_ERR_BUDGET, "no free scenario slot");
     const size_t need = thy_footprint(ctx->nx, ctx->ny, &ctx->prm);
     if (b->budget > 0 && b->used + need > b->budget) return bfail(b, THY_ERR_BUDGET, "HBM budget exceeded");
-    int r;
     thy_run_result res;
-    if ((r = prepare_run(ctx, cfg, &res)) != THY_OK) return bfail(b, r, thy_last_error(ctx));
-    Ctrl cb;
-    if ((r = read_ctrl(ctx, &cb)) != THY_OK) return bfail(b, r, thy_last_error(ctx));
-    bool changed;
-    if ((r = batch_faults(b, ctx, cb, changed)) != THY_OK) return r;
+    if (cfg) {
+        const int r = prepare_for_batch(ctx, cfg, res);
+        if (r != THY_OK) return bfail(b, r, thy_last_error(ctx));
+    } else {
+        res = ctx->prep_res;
+    }
+    ctx->batch_ready = false;
     const StepArgs<float> a = make_args<float>(ctx);
     // ordered after a launch in flight, which does not read this free slot
     b->h_args[slot] = a;

@@ -99,6 +99,7 @@ def lib():
         L.thy_copy_probe.argtypes = [C.c_size_t, C.c_int, _dp]
         L.thy_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
         L.thy_batch_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
+        L.thy_batch_prepare.argtypes = [C.c_void_p, C.POINTER(RunConfig)]
         L.thy_batch_destroy.argtypes = [C.c_void_p]
         L.thy_batch_last_error.argtypes = [C.c_void_p]
         L.thy_batch_last_error.restype = C.c_char_p
@@ -157,6 +158,7 @@ class Solver:
     def __init__(self, prob, precision=0, track_arrival=True, tile=(0, 0), upload=True, kernel="sweep",
                  strip_rows=0, general_rows=0):
         self.prob = prob
+        self.batch_prepared = False
         self._keep = [np.ascontiguousarray(a, np.float64) for a in
                       (prob.lon, prob.lat, prob.area, prob.cosphi, prob.ancho, prob.alto)]
         g = Grid(prob.nx, prob.ny, *[a.ctypes.data_as(_dp) for a in self._keep])
@@ -194,6 +196,7 @@ class Solver:
         s1[:, 1] = np.ravel(H)
         s2[:, 0] = np.ravel(qx)
         s2[:, 1] = np.ravel(qy)
+        self.batch_prepared = False
         self._chk(lib().thy_upload_state(self.h, s1.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)), "upload")
 
     def state(self):
@@ -286,8 +289,18 @@ class Solver:
                 "g_v": bufs[3][:ns * ng].reshape(ns, ng), "g_eta_min": bufs[4][:ng], "g_eta_max": bufs[5][:ng],
                 "g_bathy": bufs[6][:ng]}
 
+    def prepare_batch(self, steps_per_sync=0, max_steps=0):
+        """The context-local part of admission to a batch (thy_batch_prepare): run
+        configuration, metrics, initial dt, faults due at t = 0.  May run on another
+        thread than the batch's; Batch.add then admits the solver as prepared."""
+        c = self.run_config(max_steps=max_steps, steps_per_sync=steps_per_sync, save_interval=-1.0)
+        self.batch_prepared = False
+        self._chk(lib().thy_batch_prepare(self.h, C.byref(c)), "batch_prepare")
+        self.batch_prepared = True
+
     def run(self, max_steps=0, steps_per_sync=0, on_snapshot=None, on_okada=None, faults=None, on_step=None):
         c = self.run_config(max_steps, steps_per_sync, on_snapshot, on_okada, faults, on_step)
+        self.batch_prepared = False
         r = RunResult()
         self._chk(lib().thy_run(self.h, C.byref(c), C.byref(r)), "run")
         out = {"steps": r.steps, "t_final": r.t_final, "dt_initial": r.dt_initial, "seconds": r.seconds,
@@ -320,13 +333,19 @@ class Batch:
             raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {msg}", rc)
 
     def add(self, solver, steps_per_sync=0, max_steps=0):
-        """Admit a scenario; returns its slot, or None when the budget or the slots are full."""
-        c = solver.run_config(max_steps=max_steps, steps_per_sync=steps_per_sync, save_interval=-1.0)
+        """Admit a scenario; returns its slot, or None when the budget or the slots are
+        full.  A solver prepared by Solver.prepare_batch is admitted as prepared (the
+        arguments here are then unused)."""
         slot = C.c_int()
-        rc = lib().thy_batch_add(self.h, solver.h, C.byref(c), C.byref(slot))
+        if solver.batch_prepared:
+            rc = lib().thy_batch_add(self.h, solver.h, None, C.byref(slot))
+        else:
+            c = solver.run_config(max_steps=max_steps, steps_per_sync=steps_per_sync, save_interval=-1.0)
+            rc = lib().thy_batch_add(self.h, solver.h, C.byref(c), C.byref(slot))
         if rc == STATUS_CODES["BUDGET"]:
             return None
         self._chk(rc, "batch_add")
+        solver.batch_prepared = False
         self.solvers[slot.value] = solver
         return slot.value
 

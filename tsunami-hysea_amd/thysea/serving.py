@@ -21,7 +21,7 @@ def _result(r):
 
 
 def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8,
-                workers=4, solver_kw=None):
+                workers=8, solver_kw=None):
     """Serve ``problems`` (arrival times in seconds, default all at 0) through one
     Batch.  Returns per-request records and, for indices in ``keep``, the results.
 
@@ -29,9 +29,9 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
     is launched asynchronously and the loop admits arrived requests and retires
     finished ones until it has completed (at most ``burst`` admissions while the GPU is
     idle, so that stepping starts early).  The work that touches one context only
-    (creating and uploading it; reading out its max height, arrival times and gauges
-    and releasing it) runs on ``workers`` threads; the batch itself is driven from
-    this thread.  ``solver_kw`` goes to every native.Solver (strip heights, for one)."""
+    (creating and uploading it, preparing its run with the faults due at t = 0;
+    reading out its max height, arrival times and gauges and releasing it) runs on
+    ``workers`` threads; the batch itself is driven from this thread.  ``solver_kw`` goes to every native.Solver (strip heights, for one)."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
     order = list(np.argsort(arrivals, kind="stable"))
@@ -51,7 +51,9 @@ def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_s
         return time.perf_counter() - t0
 
     def prepare(i):
-        return native.Solver(problems[i], **(solver_kw or {}))
+        s = native.Solver(problems[i], **(solver_kw or {}))
+        s.prepare_batch(steps_per_sync=steps_per_sync)
+        return s
 
     def readout(s, n_samples, steps):
         out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"), "steps": steps}
