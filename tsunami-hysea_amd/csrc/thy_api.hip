@@ -62,6 +62,7 @@ struct thy_ctx {
     int kernel = 0;              // 0 row sweep (deep-water + general split), 1 2D tiles, 2 row sweep general only
     int srows = 24;              // row sweep: rows per strip
     int gsrows = 32;             //   rows per general strip beside the deep-water launch
+    int yoff = 0;                //   row offset of the split launches' strip rows
     int grows_req = 0;           //   requested general strip height (0: the default rule)
     int ncu = 256;               // compute units of the device
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
@@ -167,6 +168,7 @@ StepArgs<T> make_args(thy_ctx *c) {
         a.tiles_y = (c->ny + c->srows - 1) / c->srows;
         a.srows = c->srows;
         a.gsrows = c->gsrows;
+        a.yoff = c->yoff;
         a.order = c->order;
         a.n_order = c->n_order;
         a.nseg = nseg;
@@ -358,6 +360,29 @@ int general_rows_of(int S, int req, size_t n_general, int ncu) {
     return S;
 }
 
+// Row offset of the split launches' strip rows (strip row k covers rows k*S-yoff ..
+// k*S-yoff+S-1, cut to the grid).  A strip row is deep-water capable only clear of the
+// top and bottom border bands and sponges; the offset that makes the rows that are not
+// thinnest, so that they cost the least general work (a general row about five times a
+// deep-water row, halos included), is taken; ties go to the smaller offset.
+int strip_row_offset(int ny, int S, int sps, int spn) {
+    int best = 0;
+    long best_cost = -1;
+    for (int off = 0; off < S; off++) {
+        long cost = 0;
+        for (int y = -off; y < ny; y += S) {
+            const int y0 = std::max(y, 0), y1 = std::min(y + S, ny);
+            const bool ok = y0 - 4 >= 2 && y1 + 3 <= ny - 3 && y0 - 2 >= sps && y1 + 1 < ny - spn;
+            cost += (long)(y1 - y0 + 8) * (ok ? 1 : 5);
+        }
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = off;
+        }
+    }
+    return best;
+}
+
 template <typename T>
 int build_order(thy_ctx *ctx, const T *D) {
     thy_ctx *c = ctx;
@@ -384,11 +409,16 @@ int build_order(thy_ctx *ctx, const T *D) {
     // byte range
     const bool split = c->kernel == 0 && c->prec == 0 && (size_t)P * ny * sizeof(float) <= (size_t)INT32_MAX;
     const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
-    const int n_strips = tiles_y * nseg;
+    // strips of the split launches: rows shifted up by yoff (the block order of the
+    // launch without a split keeps unshifted tiles)
+    const int yoff = split ? strip_row_offset(ny, S, sps, spn) : 0;
+    c->yoff = yoff;
+    const int strip_rows = (ny + yoff + S - 1) / S;
+    const int n_strips = strip_rows * nseg;
     std::vector<int> scls(n_strips, 0);
     std::vector<unsigned char> sdeep(n_strips, 0);
-    for (int ty = 0; ty < tiles_y; ty++) {
-        const int y0 = ty * S, y1 = std::min(y0 + S, ny);
+    for (int ty = 0; ty < strip_rows; ty++) {
+        const int y0 = std::max(ty * S - yoff, 0), y1 = std::min(ty * S - yoff + S, ny);
         const int rows = (y1 - y0) + 8;
         const bool yok = y0 - 4 >= 2 && y1 + 3 <= ny - 3 && y0 - 2 >= sps && y1 + 1 < ny - spn;
         for (int sgi = 0; sgi < nseg; sgi++) {
@@ -404,9 +434,10 @@ int build_order(thy_ctx *ctx, const T *D) {
     std::vector<int> cls(n_tiles, 0);
     for (int t = 0; t < n_tiles; t++) {
         const int ty = t / tiles_x, tx = t - ty * tiles_x;
+        const int sr = (ty * S + yoff) / S;  // the shifted strip row holding the tile's first row
         for (int w = 0; w < SW_WAVES; w++) {
             const int sgi = tx * SW_WAVES + w;
-            if (sgi < nseg) cls[t] = std::max(cls[t], scls[ty * nseg + sgi]);
+            if (sgi < nseg) cls[t] = std::max(cls[t], scls[sr * nseg + sgi]);
         }
     }
     // chunked XCD order as the base sequence (the kernel's order without a table)
@@ -431,7 +462,19 @@ int build_order(thy_ctx *ctx, const T *D) {
     std::vector<int> deep_rm, gen, gcls;
     c->bgen_h.clear();
     c->bgcls_h.clear();
-    for (int t : base) {
+    // the general strips in the chunked XCD order over the shifted strip rows
+    std::vector<int> sbase;
+    {
+        const int st = tiles_x * strip_rows;
+        const int s_chunks = (strip_rows + c->chunk_rows - 1) / c->chunk_rows;
+        const int s_grid = ((s_chunks + 7) / 8) * 8 * chunk;
+        for (int bid = 0; bid < s_grid; bid++) {
+            const int loc = bid >> 3, ch = loc / chunk;
+            const int t = ((ch << 3) + (bid & 7)) * chunk + (loc - ch * chunk);
+            if (t < st) sbase.push_back(t);
+        }
+    }
+    for (int t : sbase) {
         const int ty = t / tiles_x, tx = t - ty * tiles_x;
         for (int w = 0; w < SW_WAVES; w++) {
             const int sgi = tx * SW_WAVES + w;
@@ -440,7 +483,9 @@ int build_order(thy_ctx *ctx, const T *D) {
             if (sdeep[item]) continue;
             c->bgen_h.push_back(item);
             c->bgcls_h.push_back(scls[item]);
-            for (int sub = 0; sub < S / G && (ty * S + sub * G) < ny; sub++) {
+            for (int sub = 0; sub < S / G; sub++) {
+                const int y = ty * S + sub * G - yoff;  // the sub-strip's first row, before the cut at 0
+                if (y + G <= 0 || y >= ny) continue;
                 gen.push_back((ty * (S / G) + sub) * nseg + sgi);
                 gcls.push_back(scls[item]);
             }
@@ -485,7 +530,7 @@ int build_order(thy_ctx *ctx, const T *D) {
         c->gexec = nullptr;
         c->gk = 0;
     }
-    const int cap = std::max({n_strips, ((ny + G - 1) / G) * nseg, (int)deep.size()});
+    const int cap = std::max({n_strips, strip_rows * (S / G) * nseg, (int)deep.size()});
     if (c->list_cap < cap) {
         void *ptrs[] = {c->order, c->deep, c->gen, c->fb_list};
         for (void *p : ptrs)

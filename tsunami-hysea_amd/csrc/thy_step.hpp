@@ -57,6 +57,7 @@ template <typename T> struct StepArgs {
     int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
     int srows;                             // row-sweep kernel: rows per strip
     int gsrows;                            //   rows per general strip beside a deep-water launch
+    int yoff;                              //   split launches: strip row k covers rows k*rows-yoff ..
     int nseg;                              // row-sweep kernel: 56-column segments per row
     const int *order;                      // row-sweep kernel: every block, costliest first (or null)
     int n_order;                           //   number of them

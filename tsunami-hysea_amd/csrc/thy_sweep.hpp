@@ -416,12 +416,25 @@ __device__ __forceinline__ float sweep_block(const StepArgs<T> &a, const BlockCt
 }
 
 // wavefront work items of the split launches: strip row * nseg + segment (-1: idle),
-// strip rows in units of `rows`
+// strip rows in units of `rows`, shifted up by a.yoff rows so that the strip rows along
+// the top and bottom borders are thin (the first one is cut at row 0).  n: the strip's
+// rows inside the grid.
 template <typename T>
-__device__ __forceinline__ void item_origin(const StepArgs<T> &a, const int item, const int rows, int &x0, int &y0) {
+__device__ __forceinline__ void item_origin(const StepArgs<T> &a, const int item, const int rows, int &x0, int &y0,
+                                            int &n) {
     const int strip = item / a.nseg;
     x0 = (item - strip * a.nseg) * SW_OUT;
-    y0 = strip * rows;
+    const int y = strip * rows - a.yoff;
+    y0 = max(y, 0);
+    n = min(y + rows, a.ny) - y0;
+}
+
+// origin of a deep-water strip (never cut at row 0: it lies clear of the top border band)
+template <typename T>
+__device__ __forceinline__ void deep_origin(const StepArgs<T> &a, const int item, int &x0, int &y0) {
+    const int strip = item / a.nseg;
+    x0 = (item - strip * a.nseg) * SW_OUT;
+    y0 = strip * a.srows - a.yoff;
 }
 
 // workgroup minimum of the lanes' dt, folded into the launch's dt slot of the grid
@@ -486,9 +499,9 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
         const int rows = mode == 1 ? a.gsrows : a.srows;
         dtmin = 3.0e38f;
         if (item >= 0) {
-            int x0, y0;
-            item_origin(a, item, rows, x0, y0);
-            dtmin = sweep_strip(a, bc, x0, y0, rows);
+            int x0, y0, n;
+            item_origin(a, item, rows, x0, y0, n);
+            if (n > 0) dtmin = sweep_strip(a, bc, x0, y0, n);
         }
     }
     block_dt_min(a, slot, dtmin, s_red);
@@ -685,7 +698,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<flo
     float dtmin = 3.0e38f;
     if (item >= 0) {
         int x0, y0;
-        item_origin(a, item, a.srows, x0, y0);
+        deep_origin(a, item, x0, y0);
         if (!deep_strip(a, bc, x0, y0, dtmin)) {
             // hand the strip back; its dt comes from the general launch that redoes it
             dtmin = 3.0e38f;
@@ -756,7 +769,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_batch_deep(const BatchArgs q,
     const BlockCtl<float> bc = wave_begin(a, slot, (code & 1) != 0);
     if (!bc.run) return;
     int x0, y0;
-    item_origin(a, code >> 1, a.srows, x0, y0);
+    deep_origin(a, code >> 1, x0, y0);
     float dtmin;
     if (!deep_strip(a, bc, x0, y0, dtmin)) {
         // hand the strip back; its dt comes from the launch that redoes it
@@ -778,9 +791,9 @@ __device__ __forceinline__ void batch_general_item(const BatchArgs &q, const int
     const BlockCtl<float> bc = wave_begin(a, slot, (code & 1) != 0);
     if (!bc.run) return;
     const int rows = sub ? a.gsrows : a.srows;
-    int x0, y0;
-    item_origin(a, code >> 1, rows, x0, y0);
-    wave_dt_min(a, slot, sweep_strip(a, bc, x0, y0, rows));
+    int x0, y0, n;
+    item_origin(a, code >> 1, rows, x0, y0, n);
+    wave_dt_min(a, slot, n > 0 ? sweep_strip(a, bc, x0, y0, n) : 3.0e38f);
 }
 
 // HANDBACK false: the general strips, one wavefront each.  HANDBACK true: the strips
