@@ -47,7 +47,17 @@ class Pool {
                 return hipSuccess;
             }
         }
+        return new_streams(a, b);
+    }
+
+    // a new pair, not from the cache
+    hipError_t new_streams(hipStream_t &a, hipStream_t &b) {
         a = b = nullptr;
+        // the runtime gives a new stream the hardware queue that serves the fewest
+        // streams; the two of a pair are created back to back (no other thread's stream
+        // in between), so that unless one queue serves two streams fewer than every
+        // other, they go to two different queues and their launches overlap
+        std::lock_guard<std::mutex> g(create_m_);
         hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
         if (e != hipSuccess) {
@@ -189,6 +199,7 @@ class Pool {
     }
 
     std::mutex m_;
+    std::mutex create_m_;  // stream pairs are created one pair at a time
     Heap dev_, host_;
     std::vector<std::pair<hipStream_t, hipStream_t>> streams_;
 };
