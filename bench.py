@@ -246,7 +246,7 @@ def main():
     log("warmup done: t =", w["t_final"] * prob.T, "s")
     tuned = None
     if args.kernel == "sweep" and args.strip_rows == 0:
-        rows, tms = solver.autotune(reps=15)  # candidates differ by 1-2 %: a median of 15
+        rows, tms = solver.autotune(reps=9)  # candidates differ by 1-2 %: a median of 9 replays
         tuned = {"strip_rows": rows, "general_rows": solver.general_rows, "ms_per_step": tms}
         log(f"autotune: {rows}-row strips ({solver.general_rows}-row general strips), {tms:.4f} ms per step")
     t0 = time.perf_counter()

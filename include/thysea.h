@@ -187,9 +187,10 @@ size_t thy_footprint(int nx, int ny, const thy_params *params);
 int thy_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 /* Strip-height autotuner of the row-sweep kernel.  No reference counterpart (the
-   reference has one fixed launch shape).  Times `reps` step launches (HIP events) from
-   the current state for pairs of strip height and general strip height (16/4, 16/8,
-   24/4, 24/8, 24/12, 32/8, 32/16, 36/18, 40/20, 48/16) and keeps the fastest; the state, control
+   reference has one fixed launch shape).  For pairs of strip height and general strip
+   height (16/4, 16/8, 24/4, 24/8, 24/12, 32/8, 32/16, 36/18, 40/20, 48/16) it captures 12
+   steps from the current state into a graph (each step redoes the same step), times
+   `reps` replays (HIP events, median) and keeps the fastest; the state, control
    blocks and dt slots are left as they were.  Call after thy_upload_state, before or
    between runs.  general_rows may be null. */
 int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, double *ms_per_step);

@@ -368,7 +368,7 @@ def test_autotune_mid_run_leaves_the_run_unchanged():
     b = _native().Solver(pr)
     b.run(max_steps=12, steps_per_sync=6)
     rows, ms = b.autotune(reps=2)
-    assert rows in (16, 24, 32, 40, 48, 64) and ms > 0
+    assert rows in (16, 24, 32, 36, 40, 48) and ms > 0  # the candidate strip heights
     r = b.run_more(18)
     assert r["steps"] == 30
     for x, y in zip(_fields(a), _fields(b)):

@@ -1124,20 +1124,45 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
         ctx->grows_req = sg[1];
         r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
         if (r != THY_OK) return r;
-        std::vector<float> t;
-        for (int k = 0; k <= reps; k++) {  // the first launch is a warm-up
+        // kSeq steps captured into one graph and replayed, as a run replays its steps;
+        // every step is launched with slot 0 (launch_step advances it), so each reads the
+        // same state and control block and redoes the same step: the run's state stays
+        constexpr int kSeq = 12;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < kSeq; q++) {
             ctx->slot = 0;
-            if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
-            if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
-            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
             hipError_t e = launch_step(ctx);
-            if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("autotune launch: ") + hipGetErrorString(e));
-            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-            HIPCHK(hipEventSynchronize(ctx->ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            if (k > 0) t.push_back(ms);
+            if (e != hipSuccess) {
+                (void)hipStreamEndCapture(ctx->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                return fail(ctx, THY_ERR_HIP, std::string("autotune capture: ") + hipGetErrorString(e));
+            }
         }
+        HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+        hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+        std::vector<float> t;
+        for (int k = 0; k <= reps; k++) {  // the first replay is a warm-up
+            ctx->slot = 0;
+            hipError_t e = hipSuccess;
+            if ((r = write_ctrl(ctx, c)) == THY_OK && ctx->fb_count)
+                e = ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int));
+            if (e == hipSuccess && r == THY_OK) e = hipEventRecord(ctx->ev0, ctx->stream);
+            if (e == hipSuccess && r == THY_OK) e = hipGraphLaunch(ge, ctx->stream);
+            if (e == hipSuccess && r == THY_OK) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess && r == THY_OK) e = hipEventSynchronize(ctx->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess && r == THY_OK) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+            if (e != hipSuccess || r != THY_OK) {
+                (void)hipGraphExecDestroy(ge);
+                return r != THY_OK ? r : fail(ctx, THY_ERR_HIP, std::string("autotune replay: ") + hipGetErrorString(e));
+            }
+            if (k > 0) t.push_back(ms / kSeq);
+        }
+        (void)hipGraphExecDestroy(ge);
         std::sort(t.begin(), t.end());
         const double med = t[t.size() / 2];
         if (med < best_ms) {
