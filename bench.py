@@ -7,12 +7,15 @@ seamounts, coastline along the east and north edges, four 100x50 km subfaults wi
 2 m slip applied at t = 0, 64 gauges).  State is resident in HBM before the timed
 region starts.  Prints one JSON line (rank 0, N = 1):
 
-  value        Gcell-steps/s over exactly --steps steps after --warmup steps
-  roofline     the step (row sweep: deep-water launch beside the general launch, then the
-               hand-back launch): algorithmic bytes per step / measured step time (HIP
-               events on the library's stream around each graph replay) vs 8 TB/s
-  cpu_baseline FP64 restatement of the reference scheme (the oracle) timed on the
-               host cores for a bounded sample of the same workload
+  value        Gcell-steps/s over exactly --steps steps after --warmup steps: host wall
+               time of one thy_run_more call that replays one captured sequence of
+               exactly --steps steps (strip heights tuned and the sequence captured and
+               uploaded before the clock starts)
+  roofline     44 algorithmic bytes per cell-step at that rate vs 8 TB/s (the same time
+               as value); kernel_ms is the HIP-event time of the replay per step
+  cpu_baseline FP64 restatement of the reference scheme (the oracle) timed on all host
+               cores for a bounded sample of the same workload, plus the 240x200 workload
+               run in full
 """
 import argparse
 import json
@@ -38,15 +41,36 @@ def log(*a):
 
 
 def cpu_model():
-    """Model name of the host CPU (Linux), or None."""
+    """Model name of the host CPU with its family/model/stepping numbers (Linux), or None."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    if name is None:
+        return None
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}, {os.cpu_count()} logical CPUs)")
+
+
+def cpu_full_run(threads):
+    """The 240x200 workload (3600 s, one fault, 16 gauges) run in full by the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from thysea import scenario
+    pr = scenario.problem("small")
+    s = O.OracleSim(pr, threads=threads)
+    t0 = time.perf_counter()
+    r = s.run()
+    sec = time.perf_counter() - t0
+    s.close()
+    return {"workload": f"{pr.nx}x{pr.ny}, {pr.t_end * pr.T:.0f} s, whole driver loop", "steps": int(r["steps"]),
+            "seconds": sec, "gcell_steps_per_s": pr.nx * pr.ny * r["steps"] / sec / 1e9, "cores": threads}
 
 
 def cpu_baseline(prob, steps, threads):
@@ -79,16 +103,20 @@ def batched_phase(n, sample, sync):
     rows_b, cost_b = serving.tune_batch(probs, steps_per_sync=sync)
     log("batched: strip heights " + ", ".join(f"{k}: {v:.2f}" for k, v in cost_b.items())
         + f" us per scenario-step; {rows_b} rows")
+    # the sequential sample first, so that nothing the batch phase leaves behind can
+    # slow the denominator of the speedup; each scenario timed on its own
+    t0 = time.perf_counter()
+    rec_s, res_s = serving.run_sequential([probs[i] for i in keep], steps_per_sync=sync, keep=keep)
+    wall_s = time.perf_counter() - t0
+    per = np.array([r["finished"] - r["admitted"] for r in rec_s])
+    per_s = wall_s / len(keep)
+    log(f"sequential: {len(keep)} scenarios in {wall_s:.2f} s ({per_s*1e3:.1f} ms each; "
+        f"min {per.min()*1e3:.1f}, median {np.median(per)*1e3:.1f}, max {per.max()*1e3:.1f} ms)")
     t0 = time.perf_counter()
     rec_b, res_b, gpu_b = serving.run_batched(probs, steps_per_sync=sync, keep=keep,
                                               solver_kw=dict(strip_rows=rows_b))
     wall_b = time.perf_counter() - t0
     log(f"batched: {wall_b:.2f} s wall, {gpu_b:.2f} s in step launches")
-    t0 = time.perf_counter()
-    rec_s, res_s = serving.run_sequential([probs[i] for i in keep], steps_per_sync=sync, keep=keep)
-    wall_s = time.perf_counter() - t0
-    per_s = wall_s / len(keep)
-    log(f"sequential: {len(keep)} scenarios in {wall_s:.2f} s ({per_s*1e3:.1f} ms each)")
     err = 0.0
     for i in keep:
         a, b = res_b[i]["max_height"], res_s[i]["max_height"]
@@ -105,7 +133,10 @@ def batched_phase(n, sample, sync):
             "speedup_vs_sequential": (n / wall_b) / (1.0 / per_s),
             "gcell_steps_per_s": cells * steps / wall_b / 1e9,
             "p50_s": st["p50_s"], "p99_s": st["p99_s"], "wall_s": wall_b,
-            "sequential_sample": len(keep), "max_height_rel_err_vs_plain": err, "strip_rows": rows_b,
+            "sequential_sample": len(keep),
+            "sequential_ms_per_scenario": {"min": float(per.min() * 1e3), "median": float(np.median(per) * 1e3),
+                                           "max": float(per.max() * 1e3)},
+            "max_height_rel_err_vs_plain": err, "strip_rows": rows_b,
             "strip_rows_tuning_us_per_scenario_step": {str(k): round(v, 3) for k, v in cost_b.items()}}
 
 
@@ -202,7 +233,10 @@ def main():
     ap.add_argument("--kernel", default="sweep", choices=["sweep", "tile"], help="step kernel")
     ap.add_argument("--strip-rows", type=int, default=0, help="row-sweep kernel: rows per strip (0: default)")
     ap.add_argument("--tile", default="0x0", help="2D-tile kernel: tile shape (0x0: default)")
-    ap.add_argument("--sync-every", type=int, default=48, help="steps per captured graph replay")
+    ap.add_argument("--sync-every", type=int, default=0,
+                    help="steps per captured graph replay (0: --steps, up to 960)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-everywhere 12M measurement")
+    ap.add_argument("--batch-sync", type=int, default=48, help="steps per captured sequence of the batch phases")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps-single", type=int, default=2, help="oracle steps timed on one thread (0: skip)")
@@ -214,6 +248,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving arrival window (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--no-serve-sequential", action="store_true")
+    ap.add_argument("--saturated-rate", type=float, default=16.0,
+                    help="arrival rate of the saturated serving run, above what one at a time sustains (0: skip)")
+    ap.add_argument("--saturated-seconds", type=float, default=10.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
     args = ap.parse_args()
 
@@ -227,8 +264,9 @@ def main():
         raise SystemExit("bench.py: no gfx950 device visible")
     log("generating workload", args.nx, "x", args.ny)
     prob = scenario.problem("large", nx=args.nx, ny=args.ny)
-    # the configured 1800 s is ~500 steps; keep stepping past it when --warmup + --steps asks for more
-    prob.t_end = max(prob.t_end, 1e9)
+    # the configured 1800 s is ~500 steps; keep stepping past it when --warmup + --steps
+    # asks for more (about 4 s per step; the gauge buffers are sized by this end time)
+    prob.t_end = max(prob.t_end, (args.warmup + args.steps + 100) * 10.0 / prob.T)
     track = not args.no_arrival
     tile = tuple(int(v) for v in args.tile.split("x"))
     ncell = prob.nx * prob.ny
@@ -239,16 +277,22 @@ def main():
         copy = {"bytes": args.copy_probe_mib << 20, "ms": ms, "GBs_read_plus_write": gbs}
         log(f"copy probe {args.copy_probe_mib} MiB: {ms:.3f} ms, {gbs:.0f} GB/s")
 
+    # one captured sequence of exactly --steps steps (any count: the library keeps one
+    # sequence per launch slot it can start at), so no idle launch enters the window
+    sync = args.sync_every if args.sync_every > 0 else (args.steps if args.steps <= 960 else 48)
     solver = native.Solver(prob, precision=0, track_arrival=track, tile=tile, kernel=args.kernel,
                            strip_rows=args.strip_rows)
     log("warmup", args.warmup, "steps (includes Okada at t=0)")
-    w = solver.run(max_steps=args.warmup, steps_per_sync=args.sync_every)
+    w = solver.run(max_steps=args.warmup, steps_per_sync=sync)
     log("warmup done: t =", w["t_final"] * prob.T, "s")
     tuned = None
     if args.kernel == "sweep" and args.strip_rows == 0:
         rows, tms = solver.autotune(reps=9)  # candidates differ by 1-2 %: a median of 9 replays
         tuned = {"strip_rows": rows, "general_rows": solver.general_rows, "ms_per_step": tms}
         log(f"autotune: {rows}-row strips ({solver.general_rows}-row general strips), {tms:.4f} ms per step")
+    # capture and upload the step sequences of the tuned strips before the clock starts
+    # (no step is taken: the run is at its step limit)
+    solver.run_more(0)
     t0 = time.perf_counter()
     r = solver.run_more(args.steps)
     wall = time.perf_counter() - t0
@@ -258,7 +302,7 @@ def main():
     gpu_s = r["seconds"]
     value = ncell * args.steps / wall / 1e9
     kern_ms = gpu_s * 1e3 / args.steps
-    achieved = B_ALG[track] * ncell / (kern_ms * 1e-3) / 1e9
+    achieved = B_ALG[track] * value  # GB/s: algorithmic bytes per cell-step at the measured rate
     log(f"timed {args.steps} steps: wall {wall*1e3:.2f} ms, gpu {gpu_s*1e3:.2f} ms, {value:.1f} Gcell-steps/s")
 
     traffic = None
@@ -274,24 +318,46 @@ def main():
     regime = {"wet_fraction": float(np.mean(wet)),
               "wet_h_le_hpos_fraction": float(np.mean(wet & (h <= prob.hpos)))}
     del h, wet
+    solver.close()
+
+    # the same workload at the reference's precision (FP64 state and arithmetic)
+    fp64 = None
+    if not args.no_fp64:
+        s64 = native.Solver(prob, precision=1, track_arrival=track)
+        w64 = s64.run(max_steps=args.warmup, steps_per_sync=sync)
+        s64.run_more(0)
+        t0 = time.perf_counter()
+        r64 = s64.run_more(args.steps)
+        wall64 = time.perf_counter() - t0
+        assert r64["steps"] - w64["steps"] == args.steps
+        v64 = ncell * args.steps / wall64 / 1e9
+        fp64 = {"value": v64, "unit": "Gcell-steps/s", "ms_per_step": wall64 * 1e3 / args.steps,
+                "bytes_per_cell_step": 2 * B_ALG[track], "roofline_frac": 2 * B_ALG[track] * v64 / HBM_PEAK_GBS,
+                "kernel": "k_sweep<double> (general faces)"}
+        log(f"fp64: {v64:.1f} Gcell-steps/s")
+        s64.close()
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or (os.cpu_count() or 1)
         log("cpu baseline:", args.cpu_steps, "oracle steps on", threads, "threads")
         gc, sec = cpu_baseline(prob, args.cpu_steps, threads)
         cpu = {"value": gc, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {prob.nx}x{prob.ny} grid after Okada "
-                         f"({sec:.1f} s), FP64 oracle, OpenMP",
+                         f"({sec:.1f} s), FP64 oracle, OpenMP on every host core",
                "cpu_model": cpu_model()}
         if args.cpu_steps_single > 0:
             g1, s1 = cpu_baseline(prob, args.cpu_steps_single, 1)
             cpu["single_thread"] = {"value": g1, "cores": 1,
                                     "sample": f"{args.cpu_steps_single} steps of the same grid ({s1:.1f} s)"}
-    solver.close()
-    batched = batched_phase(args.batch, args.batch_sample, args.sync_every) if args.batch > 0 else None
-    serve = (serving_phase(args.serve_rate, args.serve_seconds, args.sync_every, not args.no_serve_sequential)
+        cpu["small_workload_full_run"] = cpu_full_run(threads)
+        log("cpu: 240x200 full run", json.dumps(cpu["small_workload_full_run"]))
+    batched = batched_phase(args.batch, args.batch_sample, args.batch_sync) if args.batch > 0 else None
+    serve = (serving_phase(args.serve_rate, args.serve_seconds, args.batch_sync, not args.no_serve_sequential)
              if args.serve_seconds > 0 else None)
+    saturated = (serving_phase(args.saturated_rate, args.saturated_seconds, args.batch_sync,
+                               not args.no_serve_sequential)
+                 if args.saturated_rate > 0 and args.saturated_seconds > 0 else None)
     prec = None if args.no_precision else precision_phase()
     out = {
         "metric": METRIC, "value": value, "unit": "Gcell-steps/s", "n_gpus": 1,
@@ -300,13 +366,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{prob.nx}x{prob.ny} lon/lat grid ({ncell/1e6:.1f}M cells, 2 arc-min), "
                                "4 Okada subfaults, 64 gauges, open borders",
-                   "cells": ncell, "track_arrival": track, "steps_per_sync": args.sync_every,
+                   "cells": ncell, "track_arrival": track, "steps_per_sync": sync,
                    "step_kernel": args.kernel, "autotune": tuned, "cell_regime": regime,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": KERNEL_NAME[args.kernel], "kernel_ms": kern_ms,
+                     "kernel_ms_basis": "HIP events around the graph replay on the library stream, per step",
                      "bytes_per_cell_step": B_ALG[track]},
+        "fp64": fp64,
         "cpu_baseline": cpu,
     }
     if batched:
@@ -315,6 +383,8 @@ def main():
         out["precision_study"] = prec
     if serve:
         out["serving"] = serve
+    if saturated:
+        out["serving_saturated"] = saturated
     if copy:
         out["copy_probe"] = copy
     print(json.dumps(out), flush=True)

@@ -144,7 +144,7 @@ int thy_init_metrics(thy_ctx *ctx);
    (src/GPU/Metrics.cu:23-42,57-77). */
 int thy_update_metrics(thy_ctx *ctx, double t);
 
-/* Okada standard source (src/GPU/Deformacion.cu:197-222): FP64 displacement,
+/* Okada standard source (src/GPU/Deformacion.cu:118-143; crop :132-142): FP64 displacement,
    crop, H -= U_Z, eta_ini += U_Z.  def_out (optional, host) receives U_Z/H. */
 int thy_apply_okada(thy_ctx *ctx, const thy_fault *fault, int crop_flag, double crop_value,
                     double *def_out);
@@ -216,7 +216,9 @@ int thy_last_timing(thy_ctx *ctx, double *ms_step_kernel);
    or destroy other contexts, while the GPU steps the batch. */
 typedef struct thy_batch thy_batch;
 
-enum thy_slot_state { THY_SLOT_FREE = 0, THY_SLOT_RUNNING = 1, THY_SLOT_FINISHED = 2 };
+/* THY_SLOT_FAILED: the health check found NaN/Inf in the scenario's state; it was
+   retired (thy_last_error of its context says after which step) and the batch goes on. */
+enum thy_slot_state { THY_SLOT_FREE = 0, THY_SLOT_RUNNING = 1, THY_SLOT_FINISHED = 2, THY_SLOT_FAILED = 3 };
 
 /* capacity scenario slots; hbm_budget bytes of scenario state (0: unlimited);
    steps_per_sync launches per captured step sequence (rounded up to a multiple of 3). */

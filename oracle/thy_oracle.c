@@ -1376,7 +1376,7 @@ singular:
     *ux = *uy = *uz = 0.0;
 }
 
-/* src/GPU/Deformacion.cu:134-222 */
+/* src/GPU/Deformacion.cu:118-143 */
 void orc_okada(orc_sim *s, double lon0, double dlon, double lat0, double dlat,
                double lon_c, double lat_c, double depth_c, double length, double width,
                double strike, double dip, double rake, double slip,
@@ -1406,7 +1406,7 @@ void orc_okada(orc_sim *s, double lon0, double dlon, double lat0, double dlat,
     }
     if (def_out) memcpy(def_out, def, sizeof(double) * (size_t)n);
     if (crop_flag == 1) {
-        /* relative crop: float reduction of -|U_Z| (src/GPU/Deformacion.cu:211-215) */
+        /* relative crop: float reduction of -|U_Z| (src/GPU/Deformacion.cu:132-136, kernel :13-20) */
         float *neg = (float *)malloc(sizeof(float) * (size_t)n);
         for (int64_t c = 0; c < n; c++) neg[c] = (float)(-fabs(def[c]));
         float dmax = -orc_reduce_min_f(neg, n, s->p.quirk_reduction);
@@ -1416,7 +1416,7 @@ void orc_okada(orc_sim *s, double lon0, double dlon, double lat0, double dlat,
     } else if (crop_flag == 2) {
         for (int64_t c = 0; c < n; c++) if (fabs(def[c]) < crop_value) def[c] = 0.0;
     }
-    /* src/GPU/Deformacion.cu:117-132 */
+    /* src/GPU/Deformacion.cu:22-53 */
     for (int64_t c = 0; c < n; c++) {
         s->H[c] -= def[c];
         s->eta_ini[c] += def[c];

@@ -77,7 +77,7 @@ double orc_momentum_pass(orc_sim *s, double dt);
 double orc_reduce_min(const double *v, int64_t n, int quirk);
 float orc_reduce_min_f(const float *v, int64_t n, int quirk);
 
-/* Okada standard source, src/GPU/Deformacion.cu:134-222.  lon0/dlon/lat0/dlat give
+/* Okada standard source, src/GPU/Deformacion.cu:118-143.  lon0/dlon/lat0/dlat give
    the uniform coordinates lon0 + i*dlon (src/GPU/ShallowWater.cu:215-218).
    Writes U_Z/H into def_out[N] when non-NULL (before crop). */
 void orc_okada(orc_sim *s, double lon0, double dlon, double lat0, double dlat,

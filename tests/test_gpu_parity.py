@@ -72,7 +72,7 @@ def _replay(pr, kw, steps=40):
     s.okada(f)
     o.okada(f)
     dt = o.initial_dt()
-    for _ in range(40):
+    for _ in range(steps):
         dnext = o.step(dt)
         s.step(dt)
         dt = dnext
@@ -143,10 +143,10 @@ def test_full_run_small_matches_oracle(kw):
     wet = mo > -9000
     peak = np.max(np.abs(mo[wet]))
     assert np.array_equal(mg > -9000, wet)
-    assert rel_linf(mg[wet], mo[wet], peak) < 1e-3
+    assert rel_linf(mg[wet], mo[wet], peak) < 1e-4
     n = min(len(rg["g_time"]), len(ro["g_time"]))
     assert n > 10
-    assert rel_linf(rg["g_eta"][:n], ro["g_eta"][:n], peak) < 2e-3
+    assert rel_linf(rg["g_eta"][:n], ro["g_eta"][:n], peak) < 1e-4
 
 
 def _scenario(nx, ny, seed, fault_frac, slip, t_sim):
@@ -464,7 +464,10 @@ def test_deep_split_hands_back_failing_blocks():
     peak = np.max(np.abs(qxo)) + np.max(np.abs(qyo))
     assert rel_linf(qxg, qxo, peak) < 1e-4 and rel_linf(qyg, qyo, peak) < 1e-4
     wet = ho > pr.eps_h
-    assert rel_linf(hg[wet], ho[wet], np.max(np.abs((ho - Ho)[wet]))) < 1e-3
+    assert np.array_equal(hg > pr.eps_h, wet)
+    # eta = h - H - Hmin on wet cells, relative to its peak (the dry hole fills from its rim)
+    eta_o, eta_g = (ho - Ho - pr.hmin)[wet], (hg - Hg - pr.hmin)[wet]
+    assert rel_linf(eta_g, eta_o, np.max(np.abs(eta_o))) < 1e-4
     s.close()
 
 

@@ -31,6 +31,16 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+def test_shallow_water_library_exports_the_reference_boundary():
+    """libthysea_sw.so exports shallowWater, declared in include/thysea_sw.h with the
+    reference's 53 parameters (src/GPU/ShallowWater.cu:182-190)."""
+    src = re.sub(r"//[^\n]*", "", open(os.path.join(ROOT, "include", "thysea_sw.h")).read())
+    decl = re.search(r'extern "C" int shallowWater\((.*?)\);', src, flags=re.S).group(1)
+    assert len([a for a in decl.split(",") if a.strip()]) == 53
+    lib = ctypes.CDLL(os.path.join(PKG, "libthysea_sw.so"))
+    assert hasattr(lib, "shallowWater")
+
+
 def _scenario(tmp_path, **kw):
     from thysea import scenario
     return scenario.make("small", str(tmp_path), **kw)

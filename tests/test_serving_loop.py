@@ -1,224 +1,177 @@
-"""Host logic of the request loop (thysea/serving.py) on the CPU.
+"""Host logic of the C++ request loop (tsunami-hysea_amd/host/serve.cpp,
+include/thysea_serve.h) on the CPU.
 
-The loop is driven against a simulated batch that keeps the contract of
-thy_batch_* (include/thysea.h): a scenario admitted while a launch is in flight joins
-the next launch; add returns no slot when the slots or the HBM budget are full; the
-batch is driven from one thread while contexts are created and read out on worker
-threads.  Every request must finish with its own results, in arrival order of
-admission, never above the budget.
+The loop is compiled here against a CPU stand-in of libthysea.so
+(tests/fake_thysea/fake_thysea.cpp) that keeps the contract of the context and batch
+entry points it calls: a scenario admitted while a launch is in flight joins the next
+launch; add refuses when the slots or the HBM budget are full; the batch must be driven
+from one thread while contexts are set up and read out on worker threads.  Every
+request must finish with its own results, admitted in arrival order, never above the
+budget; runs are prepared on the worker threads.
 """
-import threading
-import time
-import types
+import ctypes as C
+import os
+import subprocess
 
 import numpy as np
 import pytest
 
-from thysea import serving
+from conftest import ROOT
 
-RUNNING, FINISHED = 1, 2
-
-
-class FakeError(RuntimeError):
-    pass
+PKG = os.path.join(ROOT, "tsunami-hysea_amd")
 
 
-class FakeSolver:
-    made = []
-    prepared_on = []
-
-    def __init__(self, prob, strip_rows=0, general_rows=0):
-        time.sleep(0.001)  # set-up and upload take a while
-        self.prob = prob
-        self.rows = strip_rows
-        self.steps = 0
-        self.closed = False
-        self.batch_prepared = False
-        FakeSolver.made.append(self)
-
-    def prepare_batch(self, steps_per_sync=0, max_steps=0):
-        FakeSolver.prepared_on.append(threading.get_ident())
-        self.batch_prepared = True
-
-    def field(self, name):
-        off = 0.0 if name == "max_height" else 0.5
-        return np.full((3, 2), self.prob["id"] + off, np.float32)
-
-    def gauges(self, n_samples):
-        return {"n_samples": n_samples, "g_eta": np.full((n_samples, 2), self.prob["id"], np.float32)}
-
-    def run(self, steps_per_sync=0):
-        self.steps = self.prob["steps"]
-        return {"steps": self.steps, "g_eta": self.gauges(self.steps // 10)["g_eta"]}
-
-    def close(self):
-        assert not self.closed
-        self.closed = True
+@pytest.fixture(scope="module")
+def loop(tmp_path_factory):
+    d = tmp_path_factory.mktemp("fakeloop")
+    fake = d / "libthysea_fake.so"
+    serve = d / "libthysea_fakeserve.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-pthread",
+                    os.path.join(ROOT, "tests", "fake_thysea", "fake_thysea.cpp"), "-o", str(fake)], check=True)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-pthread", os.path.join(PKG, "host", "serve.cpp"),
+                    "-o", str(serve), f"-L{d}", "-lthysea_fake", f"-Wl,-rpath,{d}"], check=True)
+    from thysea import native
+    F = C.CDLL(str(fake))
+    S = C.CDLL(str(serve))
+    S.thy_serve_batched.argtypes = [C.POINTER(native.Request), C.c_int, C.POINTER(native.ServeOptions),
+                                    C.POINTER(native.RequestResult), C.POINTER(C.c_double)]
+    S.thy_serve_sequential.argtypes = [C.POINTER(native.Request), C.c_int, C.POINTER(native.ServeOptions),
+                                       C.POINTER(native.RequestResult)]
+    F.fake_stats.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
+                             C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    return native, F, S
 
 
-class FakeBatch:
-    """Slots advance by K steps in each launch they were admitted before."""
+class Reqs:
+    """Requests of the stand-in: id in lon[0], steps in t_end, footprint nx * ny bytes
+    with nx = 2."""
 
-    def __init__(self, capacity, hbm_budget=0, steps_per_sync=48):
-        self.cap, self.budget, self.K = capacity, hbm_budget, steps_per_sync
-        self.slots = {}
-        self.used = 0
-        self.max_used = 0
-        self.max_live = 0
-        self.flight = None
-        self.polls = 0
-        self.thread = threading.get_ident()
-        self.admitted = []
-        self.unprepared = 0
-        self.gpu = 0.0
+    def __init__(self, native, specs, arrivals=None):
+        n = len(specs)
+        self.n = n
+        self.req = (native.Request * n)()
+        self.out = (native.RequestResult * n)()
+        self.keep = []
+        self.res = []
+        self.specs = specs
+        for i, (steps, ny) in enumerate(specs):
+            q = self.req[i]
+            q.arrival_s = 0.0 if arrivals is None else float(arrivals[i])
+            lon = np.array([float(i), float(i) + 1.0])
+            geo = np.zeros(ny + 1)
+            q.grid = native.Grid(2, ny, lon.ctypes.data_as(native._dp), *[geo.ctypes.data_as(native._dp)] * 5)
+            q.run.t_end = float(steps)
+            gpos = np.zeros(2, np.int32)
+            q.run.n_gauges, q.run.gauge_pos = 2, gpos.ctypes.data_as(native._ip)
+            ns = steps // 10 + 1
+            b = {"max_height": np.zeros(2 * ny, np.float32), "arrival_times": np.zeros(2 * ny, np.float32),
+                 "g_time": np.zeros(ns, np.float32), "g_eta": np.zeros((ns, 2), np.float32),
+                 "g_u": np.zeros((ns, 2), np.float32), "g_v": np.zeros((ns, 2), np.float32)}
+            for k, v in b.items():
+                setattr(self.out[i], k, v.ctypes.data_as(native._fp))
+            self.out[i].gauge_capacity = ns
+            self.keep.append((lon, geo, gpos))
+            self.res.append(b)
 
-    def _main(self):
-        assert threading.get_ident() == self.thread, "the batch is driven from one thread"
-
-    def add(self, s, steps_per_sync=0, max_steps=0):
-        self._main()
-        assert not s.closed
-        if len(self.slots) >= self.cap or (self.budget and self.used + s.prob["bytes"] > self.budget):
-            return None
-        slot = min(set(range(self.cap)) - set(self.slots))
-        self.slots[slot] = [s, RUNNING]
-        self.unprepared += 0 if s.batch_prepared else 1
-        s.batch_prepared = False
-        self.used += s.prob["bytes"]
-        self.max_used = max(self.max_used, self.used)
-        self.max_live = max(self.max_live, len(self.slots))
-        self.admitted.append(s.prob["id"])
-        return slot
-
-    def launch(self):
-        self._main()
-        assert self.flight is None
-        self.flight = [k for k, v in self.slots.items() if v[1] == RUNNING]
-        assert self.flight, "a launch with nothing running"
-        self.polls = 2
-
-    def ready(self):
-        self._main()
-        if self.flight is None:
-            return True
-        self.polls -= 1
-        return self.polls <= 0
-
-    def wait(self):
-        self._main()
-        for k in self.flight:
-            s = self.slots[k][0]
-            n0 = s.steps
-            s.steps = min(s.prob["steps"], s.steps + self.K)
-            self.gpu += (s.steps - n0) * (abs(s.rows - 40) + 1) * 1e-6  # 40-row strips are the fastest
-            if s.steps == s.prob["steps"]:
-                self.slots[k][1] = FINISHED
-        self.flight = None
-
-    def status(self, slot):
-        self._main()
-        s, st = self.slots[slot]
-        return st, {"steps": s.steps, "n_samples": s.steps // 10}
-
-    def remove(self, slot):
-        self._main()
-        assert self.flight is None or slot not in self.flight
-        s, _ = self.slots.pop(slot)
-        self.used -= s.prob["bytes"]
-        return s
-
-    def advance(self):
-        self.launch()
-        self.wait()
-        return sum(v[1] == RUNNING for v in self.slots.values())
-
-    def info(self):
-        return {"gpu_seconds": self.gpu}
-
-    def close(self):
-        self._main()
+    def check(self):
+        for i, (steps, _) in enumerate(self.specs):
+            o = self.out[i]
+            assert o.status == 0, (i, o.status)
+            assert o.steps == steps and o.n_samples == steps // 10
+            b = self.res[i]
+            assert np.all(b["max_height"] == i) and np.all(b["arrival_times"] == i + 0.5)
+            assert np.all(b["g_eta"][:steps // 10] == i)
+            assert self.req[i].arrival_s <= o.admitted_s <= o.finished_s
 
 
-@pytest.fixture
-def fake(monkeypatch):
-    batches = []
-
-    class Batch(FakeBatch):
-        def __init__(self, *a, **k):
-            super().__init__(*a, **k)
-            batches.append(self)
-
-    ns = types.SimpleNamespace(Batch=Batch, Solver=FakeSolver, ThyError=FakeError, SLOT_FREE=0,
-                               SLOT_RUNNING=RUNNING, SLOT_FINISHED=FINISHED)
-    monkeypatch.setattr(serving, "native", ns)
-    return batches
+def _opts(native, capacity=6, budget=0, K=48, workers=4, burst=8):
+    o = native.ServeOptions()
+    o.capacity, o.hbm_budget, o.steps_per_sync, o.workers, o.burst = capacity, budget, K, workers, burst
+    return o
 
 
-def _problems(n, seed=3):
+def _stats(F, n):
+    adm = (C.c_double * max(1, n))()
+    ml, mp, po, un = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    mu = C.c_size_t()
+    k = F.fake_stats(adm, n, C.byref(ml), C.byref(mu), C.byref(mp), C.byref(po), C.byref(un))
+    return dict(admitted=[int(x) for x in adm[:k]], max_live=ml.value, max_used=mu.value, prepared=mp.value,
+                prepared_on_driver=po.value, unprepared=un.value)
+
+
+def _specs(n, seed=3, max_ny=5):
     rng = np.random.default_rng(seed)
-    return [dict(id=i, steps=int(rng.integers(1, 400)), bytes=int(rng.integers(1, 10))) for i in range(n)]
-
-
-def _check_results(probs, rec, res):
-    for i, p in enumerate(probs):
-        r = res[i]
-        assert r["steps"] == p["steps"] == rec[i]["steps"]
-        assert np.all(r["max_height"] == i) and np.all(r["arrival_times"] == i + 0.5)
-        assert r["g_eta"].shape == (p["steps"] // 10, 2) and np.all(r["g_eta"] == i)
-        assert rec[i]["arrival"] <= rec[i]["admitted"] <= rec[i]["finished"]
+    return [(int(rng.integers(1, 400)), int(rng.integers(1, max_ny + 1))) for _ in range(n)]
 
 
 @pytest.mark.parametrize("workers", [1, 4])
-def test_every_request_finishes_with_its_own_results(fake, workers):
-    FakeSolver.prepared_on.clear()
-    probs = _problems(40)
-    rec, res, _ = serving.run_batched(probs, capacity=6, steps_per_sync=48, keep=range(40), workers=workers)
-    _check_results(probs, rec, res)
-    b = fake[0]
-    assert b.max_live <= 6 and not b.slots
-    assert b.admitted == list(range(40))  # admitted in arrival order
+def test_every_request_finishes_with_its_own_results(loop, workers):
+    native, F, S = loop
+    F.fake_reset()
+    r = Reqs(native, _specs(40))
+    gpu = C.c_double()
+    assert S.thy_serve_batched(r.req, 40, C.byref(_opts(native, workers=workers)), r.out, C.byref(gpu)) == 0
+    r.check()
+    st = _stats(F, 40)
+    assert st["max_live"] <= 6
+    assert st["admitted"] == list(range(40))  # admitted in arrival order
     # runs are prepared on the worker threads, never on the thread that drives the batch
-    assert len(FakeSolver.prepared_on) == 40 and b.thread not in FakeSolver.prepared_on
-    assert b.unprepared == 0
+    assert st["prepared"] == 40 and st["prepared_on_driver"] == 0 and st["unprepared"] == 0
+    assert gpu.value > 0
 
 
-def test_budget_bounds_the_resident_scenarios(fake):
-    probs = [dict(id=i, steps=50 + 7 * i, bytes=4) for i in range(12)]
-    rec, res, _ = serving.run_batched(probs, capacity=64, hbm_budget=9, steps_per_sync=24, keep=range(12))
-    _check_results(probs, rec, res)
-    assert fake[0].max_used <= 9 and fake[0].max_live == 2
+def test_budget_bounds_the_resident_scenarios(loop):
+    native, F, S = loop
+    F.fake_reset()
+    specs = [(50 + 7 * i, 2) for i in range(12)]  # 4 bytes each
+    r = Reqs(native, specs)
+    assert S.thy_serve_batched(r.req, 12, C.byref(_opts(native, capacity=64, budget=9, K=24)), r.out, None) == 0
+    r.check()
+    st = _stats(F, 12)
+    assert st["max_used"] <= 9 and st["max_live"] == 2
 
 
-def test_arrivals_are_respected(fake):
-    probs = _problems(10, seed=5)
+def test_arrivals_are_respected(loop):
+    native, F, S = loop
+    F.fake_reset()
     arrivals = np.linspace(0.0, 0.05, 10)[::-1].copy()  # last request first
-    rec, res, _ = serving.run_batched(probs, arrivals=arrivals, capacity=4, keep=range(10))
-    _check_results(probs, rec, res)
-    assert fake[0].admitted == list(range(10))[::-1]
+    r = Reqs(native, _specs(10, seed=5), arrivals)
+    assert S.thy_serve_batched(r.req, 10, C.byref(_opts(native, capacity=4)), r.out, None) == 0
+    r.check()
+    assert _stats(F, 10)["admitted"] == list(range(10))[::-1]
     for i in range(10):
-        assert rec[i]["admitted"] >= arrivals[i]
+        assert r.out[i].admitted_s >= arrivals[i]
 
 
-def test_a_request_above_the_budget_is_an_error(fake):
-    probs = [dict(id=0, steps=10, bytes=20)]
-    with pytest.raises(FakeError, match="does not fit"):
-        serving.run_batched(probs, hbm_budget=8)
+def test_a_request_above_the_budget_fails_alone(loop):
+    """A request whose footprint exceeds the whole budget gets THY_ERR_BUDGET; the
+    others are served."""
+    native, F, S = loop
+    F.fake_reset()
+    r = Reqs(native, [(30, 2), (40, 20), (50, 2)])
+    assert S.thy_serve_batched(r.req, 3, C.byref(_opts(native, budget=8)), r.out, None) == 0
+    assert r.out[1].status == native.STATUS_CODES["BUDGET"]
+    assert r.out[0].status == 0 and r.out[2].status == 0 and r.out[2].steps == 50
 
 
-def test_sequential_loop_returns_the_same_results(fake):
-    probs = _problems(6, seed=9)
-    rec_b, res_b, _ = serving.run_batched(probs, capacity=3, keep=range(6))
-    rec_s, res_s = serving.run_sequential(probs, keep=range(6))
-    for i in range(6):
-        assert rec_s[i]["steps"] == rec_b[i]["steps"]
-        for k in ("max_height", "arrival_times", "g_eta"):
-            assert np.array_equal(res_s[i][k], res_b[i][k])
+def test_a_failing_scenario_is_retired_with_its_status(loop):
+    """A scenario the health check stops (the stand-in fails a run of 999999 steps) ends
+    with THY_ERR_NONFINITE; the batch and the other requests go on."""
+    native, F, S = loop
+    F.fake_reset()
+    r = Reqs(native, [(60, 2), (999999, 2), (70, 2)])
+    assert S.thy_serve_batched(r.req, 3, C.byref(_opts(native, capacity=2)), r.out, None) == 0
+    assert r.out[1].status == native.STATUS_CODES["NONFINITE"]
+    assert [r.out[i].steps for i in (0, 2)] == [60, 70] and r.out[0].status == r.out[2].status == 0
 
 
-def test_batch_tuner_picks_the_cheapest_strip_height(fake):
-    FakeSolver.made.clear()
-    probs = [dict(id=i, steps=1000, bytes=1) for i in range(40)]
-    rows, cost = serving.tune_batch(probs, candidates=(24, 32, 40, 48), n=8, launches=2, steps_per_sync=12)
-    assert rows == 40 and sorted(cost) == [24, 32, 40, 48]
-    assert cost[40] == pytest.approx(1.0) and cost[24] == pytest.approx(17.0)
-    assert len(FakeSolver.made) == 32 and all(s.closed for s in FakeSolver.made)
+def test_sequential_loop_returns_the_same_results(loop):
+    native, F, S = loop
+    specs = _specs(6, seed=9)
+    rb, rs = Reqs(native, specs), Reqs(native, specs)
+    assert S.thy_serve_batched(rb.req, 6, C.byref(_opts(native, capacity=3)), rb.out, None) == 0
+    assert S.thy_serve_sequential(rs.req, 6, C.byref(_opts(native)), rs.out) == 0
+    rb.check()
+    rs.check()
+    for i in range(1, 6):  # one at a time, in arrival order
+        assert rs.out[i].admitted_s >= rs.out[i - 1].finished_s

@@ -52,9 +52,11 @@ struct thy_ctx {
     hipStream_t side = nullptr;          // general launch beside the deep-water launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // two replays in flight: start / end / control-copied events, pinned control copies
+    // two replays in flight: start / end / control-copied events, pinned copies of the
+    // control block, the raw dt slot and the health flag after each replay
     hipEvent_t rev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     Ctrl *h_snap = nullptr;
+    unsigned int *h_aux = nullptr;     // [2][2]: raw dt slot, health flag
     int slot = 0;
     int cur = 0;                 // host mirror of ctrl.cur after a sync
     int tx = 64, ty = 16;
@@ -64,6 +66,7 @@ struct thy_ctx {
     int gsrows = 32;             //   rows per general strip beside the deep-water launch
     int yoff = 0;                //   row offset of the split launches' strip rows
     int grows_req = 0;           //   requested general strip height (0: the default rule)
+    int tuning = 0;              // launches recorded for timing replays publish no control block
     int ncu = 256;               // compute units of the device
     int *order = nullptr;        // row sweep: workgroup -> strip block, costliest first
     int order_n = 0;
@@ -78,7 +81,9 @@ struct thy_ctx {
     int n_deep = 0, n_gen = 0, list_cap = 0;
     unsigned int *fb_count = nullptr;  // 3 slots
     int *fb_list = nullptr;            // [3][list_cap]
-    hipGraphExec_t gexec = nullptr;
+    unsigned int *health = nullptr;    // non-finite flag raised by the step kernels
+    // captured sequences of gk steps, one per launch slot a replay starts at
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     int gk = 0;
     double last_ms = 0.0;
     std::vector<float> g_time, g_eta, g_u, g_v;
@@ -109,6 +114,17 @@ constexpr int kMassBlocks = 1024;
 int fail(thy_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
+}
+
+// captured step sequences hold the dispatch tables, grid sizes and launch slots of the
+// time they were recorded
+void drop_graphs(thy_ctx *c) {
+    for (hipGraphExec_t &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->gk = 0;
 }
 
 // A context's copies and fills go through its own stream and are complete on return.
@@ -208,6 +224,8 @@ StepArgs<T> make_args(thy_ctx *c) {
     a.gtime = c->gtime;
     a.steplog = c->steplog;
     a.logcap = c->logcap;
+    a.health = c->health;
+    a.no_publish = c->tuning;
     return a;
 }
 
@@ -525,11 +543,7 @@ int build_order(thy_ctx *ctx, const T *D) {
     c->order_h = base;
     c->cls_h = cls;
     c->bdeep_h = deep_rm;
-    if (c->gexec) {  // a captured step sequence holds the old tables and grid sizes
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-        c->gk = 0;
-    }
+    drop_graphs(c);  // the captured step sequences hold the old tables and grid sizes
     const int cap = std::max({n_strips, strip_rows * (S / G) * nseg, (int)deep.size()});
     if (c->list_cap < cap) {
         void *ptrs[] = {c->order, c->deep, c->gen, c->fb_list};
@@ -601,6 +615,7 @@ int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
     HIPCHK(hipMemcpyAsync(ctx->qx[1], ctx->qx[0], b, hipMemcpyDeviceToDevice, q));
     HIPCHK(hipMemcpyAsync(ctx->qy[1], ctx->qy[0], b, hipMemcpyDeviceToDevice, q));
     ctx->cur = 0;
+    HIPCHK(hipMemsetAsync(ctx->health, 0, sizeof(unsigned int), q));  // a new state is healthy until stepped
     const int r = build_order<T>(ctx, D);  // on the host while the copies run
     HIPCHK(hipStreamSynchronize(q));
     return r;
@@ -764,59 +779,108 @@ int mass_t(thy_ctx *ctx, double *mass) {
     return THY_OK;
 }
 
-// Capture K consecutive step launches (slots 0,1,2,...) into one graph.
-int build_graph(thy_ctx *ctx, int K) {
-    if (ctx->gexec && ctx->gk == K) return THY_OK;
-    if (ctx->gexec) {
-        (void)hipGraphExecDestroy(ctx->gexec);
-        ctx->gexec = nullptr;
-    }
+// Capture K consecutive step launches starting at launch slot s0 (slots s0, s0+1, ...
+// mod 3) into one graph, and upload it.
+int capture_steps(thy_ctx *ctx, int K, int s0) {
     hipGraph_t g;
-    int slot_saved = ctx->slot;
-    ctx->slot = 0;
+    const int slot_saved = ctx->slot;
+    ctx->slot = s0;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < K; k++) {
         hipError_t e = launch_step(ctx);
         if (e != hipSuccess) {
             (void)hipStreamEndCapture(ctx->stream, &g);
+            ctx->slot = slot_saved;
             return fail(ctx, THY_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
         }
     }
-    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
-    HIPCHK(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    ctx->gk = K;
     ctx->slot = slot_saved;
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    hipError_t e = hipGraphInstantiate(&ctx->gexec[s0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    HIPCHK(hipGraphUpload(ctx->gexec[s0], ctx->stream));
     return THY_OK;
 }
 
-// Advance until the device stop predicate holds; graph replays of K steps.  The
-// control block after each replay is copied to pinned memory behind it.  While the
-// last replay's time, steps and mean dt say that the run goes on for two more full
-// replays (before t_end, max_steps, the next fault and the next snapshot), the next
-// replay is queued before the one in flight is waited for, so that the device does not
-// idle between replays.  A queued replay that finds the run stopped idles through its
-// launches (the stop predicate holds in every one of them).
+// The step sequences replays of K steps need from the current launch slot on: with K a
+// multiple of 3 every replay starts at the same slot, otherwise the replays cycle
+// through all three (so any step count can be replayed without idle launches).
+int build_graphs(thy_ctx *ctx, int K) {
+    if (ctx->gk != K) drop_graphs(ctx);
+    ctx->gk = K;
+    int r;
+    for (int m = 0; m < 3; m++) {
+        const int s0 = (ctx->slot + m * K) % 3;
+        if (!ctx->gexec[s0] && (r = capture_steps(ctx, K, s0)) != THY_OK) return r;
+        if (K % 3 == 0) break;
+    }
+    return THY_OK;
+}
+
+// The stop predicate holds for the control block of the next launch, as seen on the
+// host after a replay: finalise its dt on the host (truncation at fault times, as the
+// launch would have done) and write it back, instead of replaying a sequence of launches
+// that would only find the run stopped.  raw: the previous step's dt minimum.
+int finalise_on_host(thy_ctx *ctx, Ctrl &c, const unsigned int raw) {
+    if (c.dt_valid) return THY_OK;
+    float rf;
+    memcpy(&rf, &raw, 4);
+    int skip;
+    const double dt = step_dt(c, ctx->ftimes.data(), rf, skip);
+    if (c.steps > 0 && ctx->on_step) {  // the log entry the launch would have written
+        const double ent[2] = {dt, c.t};
+        HIPCHK(ctx_copy(ctx, ctx->steplog + 2 * ((c.steps - 1) % ctx->logcap), ent, sizeof(ent),
+                        hipMemcpyHostToDevice));
+    }
+    c.dt = dt;
+    c.skip = skip;
+    c.dt_valid = 1;
+    return write_ctrl(ctx, c);
+}
+
+// Advance until the stop predicate holds; graph replays of K steps.  After each replay
+// the control block, the raw dt slot and the health flag are copied to pinned memory
+// behind it.  While the last replay's time, steps and mean dt say that the run goes on
+// for two more full replays (before t_end, max_steps, the next fault and the next
+// snapshot), the next replay is queued before the one in flight is waited for, so that
+// the device does not idle between replays.  When the host sees the stop predicate hold
+// after a replay it finalises the control block itself; a queued replay that finds the
+// run stopped idles through its launches (the stop predicate holds in every one).
 int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
     int r;
-    while (ctx->slot != 0) {  // align to the graph's slot sequence
-        hipError_t e = launch_step(ctx);
-        if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, hipGetErrorString(e));
-    }
-    if ((r = build_graph(ctx, K)) != THY_OK) return r;
+    if ((r = build_graphs(ctx, K)) != THY_OK) return r;
     Ctrl last;
     if ((r = read_ctrl(ctx, &last)) != THY_OK) return r;
+    {
+        unsigned int aux[2];
+        HIPCHK(ctx_copy(ctx, &aux[0], &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost));
+        HIPCHK(ctx_copy(ctx, &aux[1], ctx->health, 4, hipMemcpyDeviceToHost));
+        if (aux[1]) last.nonfinite = 1;
+        if (must_stop(last, ctx->ftimes.data())) {  // stopped before any step
+            if ((r = finalise_on_host(ctx, last, aux[0])) != THY_OK) return r;
+            *out = last;
+            if (ms) *ms = 0.0;
+            return THY_OK;
+        }
+    }
     double total = 0.0, dt_mean = 0.0;
     int head = 0, queued = 0;
     auto enqueue = [&]() -> hipError_t {
         const int i = (head + queued) % 2;
         hipError_t e;
         if ((e = hipEventRecord(ctx->rev[i][0], ctx->stream)) != hipSuccess) return e;
-        if ((e = hipGraphLaunch(ctx->gexec, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipGraphLaunch(ctx->gexec[ctx->slot], ctx->stream)) != hipSuccess) return e;
         if ((e = hipEventRecord(ctx->rev[i][1], ctx->stream)) != hipSuccess) return e;
         ctx->slot = (ctx->slot + K) % 3;
         if ((e = hipMemcpyAsync(&ctx->h_snap[i], &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost,
                                 ctx->stream)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(&ctx->h_aux[2 * i], &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost,
+                                ctx->stream)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(&ctx->h_aux[2 * i + 1], ctx->health, 4, hipMemcpyDeviceToHost, ctx->stream)) !=
+            hipSuccess)
             return e;
         if ((e = hipEventRecord(ctx->rev[i][2], ctx->stream)) != hipSuccess) return e;
         queued++;
@@ -830,10 +894,13 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         if (c.k < c.nf && t2 * c.T >= ctx->ftimes[c.k]) return false;
         return true;
     };
+    unsigned int raw = 0;
     auto collect = [&](Ctrl &c) -> hipError_t {
         hipError_t e = hipEventSynchronize(ctx->rev[head][2]);
         if (e != hipSuccess) return e;
         c = ctx->h_snap[head];
+        raw = ctx->h_aux[2 * head];
+        if (ctx->h_aux[2 * head + 1]) c.nonfinite = 1;
         float f = 0.f;
         (void)hipEventElapsedTime(&f, ctx->rev[head][0], ctx->rev[head][1]);
         total += f;
@@ -851,9 +918,11 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         last = c;
         // After an executed step dt_valid is 0; it stays 1 only when a launch of this
         // replay found the stop predicate true (every later launch then idled).
-        if (c.dt_valid) {
+        const bool host_stop = !c.dt_valid && queued == 0 && must_stop(c, ctx->ftimes.data());
+        if (c.dt_valid || host_stop) {
             while (queued > 0) HIPCHK(collect(c));  // a queued replay idled: same control block
             ctx->cur = c.cur;
+            if (host_stop && (r = finalise_on_host(ctx, c, raw)) != THY_OK) return r;
             *out = c;
             if (ms) *ms = total;
             return THY_OK;
@@ -971,6 +1040,7 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
         HIPCHK(hipEventCreateWithFlags(&ctx->rev[i][2], hipEventDisableTiming));
     }
     HIPCHK(Pool::get().host_alloc((void **)&ctx->h_snap, 2 * sizeof(Ctrl)));
+    HIPCHK(Pool::get().host_alloc((void **)&ctx->h_aux, 4 * sizeof(unsigned int)));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
     for (int k = 0; k < 2; k++) {
         HIPCHK(dev_alloc(&ctx->e[k], b));
@@ -988,6 +1058,8 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     HIPCHK(dev_alloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
     HIPCHK(dev_alloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
     HIPCHK(dev_alloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
+    HIPCHK(dev_alloc((void **)&ctx->health, sizeof(unsigned int)));
+    HIPCHK(ctx_fill(ctx, ctx->health, sizeof(unsigned int)));
     HIPCHK(dev_alloc((void **)&ctx->mass_partial, kMassBlocks * sizeof(double)));
     HIPCHK(dev_alloc((void **)&ctx->steplog, (size_t)ctx->logcap * 2 * sizeof(double)));
     HIPCHK(dev_alloc((void **)&ctx->area_d, ctx->ny * sizeof(double)));
@@ -1034,11 +1106,11 @@ void thy_destroy(thy_ctx *ctx) {
     // no queued work may still use what goes back to the caches
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    drop_graphs(ctx);
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
                     ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
-                    ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list};
+                    ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list, ctx->health};
     for (void *q : ptrs)
         dev_free(q);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1049,6 +1121,7 @@ void thy_destroy(thy_ctx *ctx) {
         for (int j = 0; j < 3; j++)
             if (ctx->rev[i][j]) (void)hipEventDestroy(ctx->rev[i][j]);
     Pool::get().host_free(ctx->h_snap);
+    Pool::get().host_free(ctx->h_aux);
     if (ctx->stream) Pool::get().put_streams(ctx->stream, ctx->side);
     delete ctx;
 }
@@ -1124,41 +1197,46 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
         ctx->grows_req = sg[1];
         r = ctx->prec ? rebuild_order_t<double>(ctx) : rebuild_order_t<float>(ctx);
         if (r != THY_OK) return r;
-        // kSeq steps captured into one graph and replayed, as a run replays its steps;
-        // every step is launched with slot 0 (launch_step advances it), so each reads the
-        // same state and control block and redoes the same step: the run's state stays
+        // kSeq steps captured into one graph and replayed, as a run replays its steps.
+        // The launches cycle through the three launch slots as in a run (so each step's
+        // hand-back list starts empty), but publish no control block: every slot holds
+        // the same control block, each step reads the same state and redoes the same
+        // step, and the run's state stays.
         constexpr int kSeq = 12;
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
+        ctx->tuning = 1;
+        ctx->slot = 0;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         for (int q = 0; q < kSeq; q++) {
-            ctx->slot = 0;
             hipError_t e = launch_step(ctx);
             if (e != hipSuccess) {
+                ctx->tuning = 0;
                 (void)hipStreamEndCapture(ctx->stream, &g);
                 if (g) (void)hipGraphDestroy(g);
                 return fail(ctx, THY_ERR_HIP, std::string("autotune capture: ") + hipGetErrorString(e));
             }
         }
+        ctx->tuning = 0;
         HIPCHK(hipStreamEndCapture(ctx->stream, &g));
         hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIPCHK(ei);
         std::vector<float> t;
+        const Ctrl c3[3] = {c, c, c};
         for (int k = 0; k <= reps; k++) {  // the first replay is a warm-up
-            ctx->slot = 0;
             hipError_t e = hipSuccess;
-            if ((r = write_ctrl(ctx, c)) == THY_OK && ctx->fb_count)
+            if ((e = ctx_copy(ctx, ctx->ctrl, c3, sizeof(c3), hipMemcpyHostToDevice)) == hipSuccess && ctx->fb_count)
                 e = ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int));
-            if (e == hipSuccess && r == THY_OK) e = hipEventRecord(ctx->ev0, ctx->stream);
-            if (e == hipSuccess && r == THY_OK) e = hipGraphLaunch(ge, ctx->stream);
-            if (e == hipSuccess && r == THY_OK) e = hipEventRecord(ctx->ev1, ctx->stream);
-            if (e == hipSuccess && r == THY_OK) e = hipEventSynchronize(ctx->ev1);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(ge, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
             float ms = 0.f;
-            if (e == hipSuccess && r == THY_OK) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-            if (e != hipSuccess || r != THY_OK) {
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+            if (e != hipSuccess) {
                 (void)hipGraphExecDestroy(ge);
-                return r != THY_OK ? r : fail(ctx, THY_ERR_HIP, std::string("autotune replay: ") + hipGetErrorString(e));
+                return fail(ctx, THY_ERR_HIP, std::string("autotune replay: ") + hipGetErrorString(e));
             }
             if (k > 0) t.push_back(ms / kSeq);
         }
@@ -1178,6 +1256,8 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     ctx->slot = saved_slot;
     HIPCHK(ctx_copy(ctx, ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
     HIPCHK(ctx_copy(ctx, ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
+    // the hand-back counts as a run leaves them between steps (the next launch resets its own)
+    if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
     if (strip_rows) *strip_rows = best;
     if (general_rows) *general_rows = ctx->gsrows;
     if (ms_best) *ms_best = best_ms;
@@ -1203,6 +1283,9 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
     Ctrl o;
     HIPCHK(ctx_copy(ctx, &o, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
     ctx->cur = o.cur;
+    unsigned int bad = 0;
+    HIPCHK(ctx_copy(ctx, &bad, ctx->health, sizeof(bad), hipMemcpyDeviceToHost));
+    if (bad) return fail(ctx, THY_ERR_NONFINITE, "the step produced a non-finite eta, qx or qy");
     if (dt_next) {
         unsigned int v;
         HIPCHK(ctx_copy(ctx, &v, &ctx->dtmin[s], sizeof(v), hipMemcpyDeviceToHost));
@@ -1294,6 +1377,9 @@ static int run_loop(thy_ctx *ctx, thy_run_result *res) {
         // after a stop every finished iteration has its log entry; otherwise the last
         // one is completed by the next launch
         if ((r = drain_log(ctx, s.dt_valid ? s.steps : s.steps - 1)) != THY_OK) return r;
+        if (s.nonfinite)
+            return fail(ctx, THY_ERR_NONFINITE,
+                        "non-finite eta, qx or qy after step " + std::to_string((long long)s.steps));
         if (!(s.t < s.t_end) || (s.max_steps > 0 && s.steps >= s.max_steps)) {
             res->steps = s.steps;
             res->t_final = s.t;
@@ -1310,10 +1396,11 @@ static int run_loop(thy_ctx *ctx, thy_run_result *res) {
             s.k++;
             changed = true;
         }
-        // src/GPU/ShallowWater.cu:404-409: snapshot
-        if (s.t >= ctx->next_save) {
+        // src/GPU/ShallowWater.cu:404-409: snapshot, at most one per iteration
+        if (!s.snap_done && s.t >= ctx->next_save) {
             ctx->next_save += ctx->save_interval;
             s.next_save = ctx->next_save;
+            s.snap_done = 1;
             if (ctx->on_snapshot) ctx->on_snapshot(ctx->user, ctx->nsnap, s.t, ctx);
             ctx->nsnap++;
             changed = true;
@@ -1355,7 +1442,7 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
     int r;
     memset(res, 0, sizeof(*res));
     ctx->batch_ready = false;  // a run prepared before is replaced
-    ctx->K = cfg->steps_per_sync > 0 ? ((cfg->steps_per_sync + 2) / 3) * 3 : 24;
+    ctx->K = cfg->steps_per_sync > 0 ? cfg->steps_per_sync : 24;
     ctx->faults.assign(cfg->faults, cfg->faults + cfg->n_faults);
     ctx->ftimes.assign(std::max(1, cfg->n_faults), 0.0);
     for (int k = 0; k < cfg->n_faults; k++) ctx->ftimes[k] = cfg->faults[k].time;
@@ -1387,8 +1474,9 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
     if (ctx->ng) HIPCHK(ctx_copy(ctx, ctx->gpos, cfg->gauge_pos, ctx->ng * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(dev_alloc((void **)&ctx->gbuf, std::max<size_t>(1, (size_t)ctx->gcap * ctx->ng * 3) * sizeof(float)));
     HIPCHK(dev_alloc((void **)&ctx->gtime, std::max(1, ctx->gcap) * sizeof(float)));
-    if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; ctx->gk = 0; }
+    drop_graphs(ctx);
 
+    HIPCHK(ctx_fill(ctx, ctx->health, sizeof(unsigned int)));
     // src/GPU/ShallowWater.cu:305-309: metrics from the pre-deformation state
     if ((r = thy_init_metrics(ctx)) != THY_OK) return r;
     // src/GPU/ShallowWater.cu:378-391: initial dt (pre-deformation) and truncation
@@ -1410,16 +1498,9 @@ static int prepare_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *
     double dt = dt0;
     int skip = 0;
     if (cfg->n_faults > 0) {
-        const double T = ctx->prm.T;
-        if (ft[0] < 2.220446049250313e-16) {
-            // skipping variant (src/GPU/Timestep.cu:46-68)
-            int i = 1;
-            while (i < cfg->n_faults && !(std::fabs(ft[i] - ft[0]) > 2.220446049250313e-16)) i++;
-            if (i < cfg->n_faults && (0.0 + dt) * T > ft[i]) { dt = ft[i] / T - 0.0; skip = 1; }
-        } else if ((0.0 + dt) * T > ft[0]) {
-            dt = ft[0] / T - 0.0;
-            skip = 1;
-        }
+        // the skipping variant when the first fault is at t = 0 (src/GPU/ShallowWater.cu:382-391)
+        if (ft[0] < 2.220446049250313e-16) trunc_skipping(c, ft.data(), 0.0, dt, skip);
+        else trunc_plain(c, ft.data(), 0.0, dt, skip);
     }
     c.dt = dt;
     c.dt_valid = 1;
@@ -1763,6 +1844,7 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
     b->bytes[slot] = need;
     b->used += need;
     b->state[slot] = THY_SLOT_RUNNING;
+    b->launched[slot] = 0;  // not in a launch in flight: it joins the next one
     b->res[slot] = res;
     b->dirty = true;
     *slot_out = slot;
@@ -1827,6 +1909,14 @@ int thy_batch_wait(thy_batch *b, int *n_running) {
             b->res[i].steps = cb.steps;  // progress, final once it has finished
             b->res[i].t_final = cb.t;
             if (!cb.dt_valid) continue;  // stepped through the whole sequence
+            if (cb.nonfinite) {
+                // the health check stopped it: retired as failed, the batch goes on
+                c->err = "non-finite eta, qx or qy after step " + std::to_string((long long)cb.steps);
+                b->state[i] = THY_SLOT_FAILED;
+                b->dirty = true;
+                BHIPCHK(hipMemsetAsync(b->d_args + i, 0, sizeof(StepArgs<float>), b->stream));
+                continue;
+            }
             // stopped: finished, or a fault is due (the only other stop a batched run has)
             if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {
                 c->nsamples = std::min(cb.nsamples, c->gcap);
@@ -1877,7 +1967,10 @@ int thy_batch_remove(thy_batch *b, int slot) {
         BHIPCHK(hipStreamSynchronize(b->stream));
         BHIPCHK(hipMemsetAsync(b->d_args + slot, 0, sizeof(StepArgs<float>), b->stream));
     }
-    // a finished slot's argument block was cleared when it finished
+    // a finished slot's argument block was cleared when it finished; a launch in flight
+    // (already waited for) must not be collected for this slot, nor for a scenario
+    // admitted to it later
+    b->launched[slot] = 0;
     b->ctx[slot]->run_active = false;
     b->used -= b->bytes[slot];
     b->bytes[slot] = 0;

@@ -200,13 +200,13 @@ __global__ void __launch_bounds__(256) k_okada(OkadaFault f, int nx, int ny, dou
         def[c] = uz;
         m = fmaxf(m, (float)fabs(uz));
     }
-    // float max of |U_Z| for relative cropping (src/GPU/Deformacion.cu:211-214)
+    // float max of |U_Z| for relative cropping (src/GPU/Deformacion.cu:13-20,132-136)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(absmax, __float_as_uint(m));
 }
 
-// crop + add: src/GPU/Deformacion.cu:22-53,211-221
+// crop + add: src/GPU/Deformacion.cu:22-53,132-142
 template <typename T>
 __global__ void __launch_bounds__(256) k_okada_apply(const double *def, int nx, int ny, int P, int crop_flag,
                                                      double crop_value, const unsigned int *absmax, T *e0, T *e1,

@@ -40,6 +40,8 @@ struct alignas(16) Ctrl {
     int cur;             // which state buffer holds the current state
     int nsamples;
     int gcap;
+    int snap_done;       // the host took the snapshot due at t: the next launch steps
+    int nonfinite;       // a step produced NaN/Inf (the health flag seen by the launch after it)
     int pad;
 };
 
@@ -86,9 +88,11 @@ template <typename T> struct StepArgs {
     float *gtime;            // [gcap]
     double *steplog;         // [logcap][2]: (dt after truncation, t) of each finished iteration
     int logcap;
+    unsigned int *health;    // set when a step wrote a non-finite e, qx or qy; stops the run
+    int no_publish;          // timing replays: the writer leaves the control blocks as they are
 };
 
-__device__ __forceinline__ void trunc_plain(const Ctrl &c, const double *ft, double t, double &dt, int &skip) {
+__host__ __device__ __forceinline__ void trunc_plain(const Ctrl &c, const double *ft, double t, double &dt, int &skip) {
     if (c.k < c.nf) {
         if ((t + dt) * c.T > ft[c.k]) {
             dt = ft[c.k] / c.T - t;
@@ -97,7 +101,7 @@ __device__ __forceinline__ void trunc_plain(const Ctrl &c, const double *ft, dou
     }
 }
 
-__device__ __forceinline__ void trunc_skipping(const Ctrl &c, const double *ft, double t, double &dt, int &skip) {
+__host__ __device__ __forceinline__ void trunc_skipping(const Ctrl &c, const double *ft, double t, double &dt, int &skip) {
     skip = 0;
     if (c.k < c.nf) {
         int i = c.k + 1;
@@ -109,13 +113,29 @@ __device__ __forceinline__ void trunc_skipping(const Ctrl &c, const double *ft, 
     }
 }
 
-// stop predicate of the driver loop, evaluated before each step
-__device__ __forceinline__ bool must_stop(const Ctrl &c, const double *ft) {
+// stop predicate of the driver loop, evaluated before each step.  A snapshot is taken
+// at most once per iteration (src/GPU/ShallowWater.cu:404-409): after the host took
+// the one due at t, the next launch steps even when t is still past the next snapshot
+// time (a dt longer than the snapshot interval).
+__host__ __device__ __forceinline__ bool must_stop(const Ctrl &c, const double *ft) {
     if (!(c.t < c.t_end)) return true;
     if (c.max_steps > 0 && c.steps >= c.max_steps) return true;
     if (c.k < c.nf && c.t * c.T >= ft[c.k]) return true;
-    if (c.t >= c.next_save) return true;
+    if (!c.snap_done && c.t >= c.next_save) return true;
+    if (c.nonfinite) return true;
     return false;
+}
+
+// dt of the launch that reads control block c: the finalised dt when c holds one, else
+// the previous step's raw minimum truncated at the next fault time
+// (src/GPU/Timestep.cu:35-68, src/GPU/ShallowWater.cu:430-436)
+__host__ __device__ __forceinline__ double step_dt(const Ctrl &c, const double *ft, const float raw, int &skip) {
+    skip = c.skip;
+    if (c.dt_valid) return c.dt;
+    double dt = (double)raw;
+    if (!skip) trunc_plain(c, ft, c.t, dt, skip);
+    else trunc_skipping(c, ft, c.t, dt, skip);
+    return dt;
 }
 
 template <typename T> struct BlockCtl {
@@ -129,16 +149,11 @@ template <typename T> struct BlockCtl {
 // the dt slot of the next launch and publishes the next control block.
 template <typename T>
 __device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot, const bool writer, BlockCtl<T> &bc) {
-    const Ctrl c = a.ctrl[slot];
-    double dt;
-    int skip = c.skip;
-    if (c.dt_valid) {
-        dt = c.dt;
-    } else {
-        dt = (double)__uint_as_float(a.dtmin[(slot + 2) % 3]);
-        if (!skip) trunc_plain(c, a.ftime, c.t, dt, skip);
-        else trunc_skipping(c, a.ftime, c.t, dt, skip);
-    }
+    Ctrl c = a.ctrl[slot];
+    // the health flag raised by the previous step's cells stops the run here
+    if (a.health && *a.health) c.nonfinite = 1;
+    int skip;
+    const double dt = step_dt(c, a.ftime, __uint_as_float(a.dtmin[(slot + 2) % 3]), skip);
     const bool stop = must_stop(c, a.ftime);
     const bool sample = !stop && (c.t >= c.next_series);
     bc.t = c.t;
@@ -150,13 +165,14 @@ __device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot,
     bc.gcap = c.gcap;
     bc.Tscale = c.T;
     if (writer) {
+        a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
+        if (a.no_publish) return;
         // iteration c.steps has finished: record the dt chosen for the next one and the time
         if (c.steps > 0 && a.steplog) {
             const int64_t li = (c.steps - 1) % a.logcap;
             a.steplog[2 * li] = dt;
             a.steplog[2 * li + 1] = c.t;
         }
-        a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
         Ctrl o = c;
         o.skip = skip;
         if (stop) {
@@ -167,6 +183,7 @@ __device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot,
             o.dt_valid = 0;
             o.steps = c.steps + 1;
             o.cur = c.cur ^ 1;
+            o.snap_done = 0;
             if (sample) {
                 o.next_series = c.next_series + c.series_dt;
                 o.nsamples = c.nsamples + 1;
@@ -201,6 +218,19 @@ __device__ __forceinline__ void gather_gauges(const StepArgs<T> &a, const BlockC
         dst[2] = vv;
     }
     if (tid == 0) a.gtime[bc.slot_sample] = (float)(bc.t * bc.Tscale);
+}
+
+// health check of a stored cell: NaN or Inf in e*, qx or qy
+template <typename T>
+__device__ __forceinline__ bool bad_cell(const T e, const T qx, const T qy) {
+    return !(__builtin_isfinite(e) && __builtin_isfinite(qx) && __builtin_isfinite(qy));
+}
+
+// a wavefront that stored a bad cell raises the context's health flag, which stops the
+// run at the next launch
+template <typename T>
+__device__ __forceinline__ void raise_health(const StepArgs<T> &a, const bool bad) {
+    if (__any(bad) && (threadIdx.x & 63) == 0 && a.health) atomicOr(a.health, 1u);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -421,6 +451,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
     const bool fast = __syncthreads_and(interior && mine) != 0;
 
     float dtmin = 3.0e38f;
+    bool bad = false;
     const T tnow = (T)bc.t;
 
     if (fast) {
@@ -536,10 +567,12 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T qy = f_qy[so] + val * ay;
                 const T es = s_es[so];
                 const int64_t go = (int64_t)gy * P + gx;
+                bad = bad || bad_cell(es, qx, qy);
                 oe[go] = es;
                 oqx[go] = qx;
                 oqy[go] = qy;
             }
+            raise_health(a, bad);
             dtmin = wave_min(dtmin);
             if ((tid & 63) == 0) s_red[tid >> 6] = dtmin;
             __syncthreads();
@@ -778,10 +811,12 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         const T es = s_es[so];
         const T hs = es + D;
         if (hs <= a.hpos) near_dry(hs, qxo, qyo, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
+        bad = bad || bad_cell(es, qx, qy);
         oe[go] = es;
         oqx[go] = qx;
         oqy[go] = qy;
     }
+    raise_health(a, bad);
     dtmin = wave_min(dtmin);
     if ((tid & 63) == 0) s_red[tid >> 6] = dtmin;
     __syncthreads();

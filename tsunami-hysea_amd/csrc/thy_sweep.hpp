@@ -163,6 +163,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
     const bool per = a.periodic != 0;
     const int y1 = min(y0 + rows, ny);
     float dtmin = 3.0e38f;
+    bool bad = false;
 
     if (x0 < nx) {
         const int gx = x0 - SW_HALO + lane;
@@ -376,6 +377,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 if (any_sponge) sponge_q(a, gx, ju, qx, qy);
                 const T hs = s4 + D;
                 if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
+                bad = bad || bad_cell(s4, qx, qy);
                 oe[go] = s4;
                 oqx[go] = qx;
                 oqy[go] = qy;
@@ -398,6 +400,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         load_metrics(rbeg - 4, pm, pt, pi);
         for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt, pi);
     }
+    raise_health(a, bad);
     return dtmin;
 }
 
@@ -548,7 +551,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
     const int P = a.pitch;
     const int y1 = min(y0 + a.srows, a.ny);
     float dtmin = 3.0e38f;
-    bool valid = true;
+    bool valid = true, bad = false;
     if (x0 < a.nx) {
         // the strip classification keeps columns x0-4 .. x0+59 and rows y0-4 .. y1+3
         // inside the grid: no clamping.  Loads are buffer loads: the lane's column is the
@@ -664,9 +667,11 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
                 dtmin = fminf(dtmin, (float)dtc);
                 const T val = dt * rg.invA;
+                const T qx = i2.qx + val * X2, qy = i2.qy + val * Y2;
+                bad = bad || bad_cell(s2, qx, qy);
                 oe[go] = s2;
-                oqx[go] = i2.qx + val * X2;
-                oqy[go] = i2.qy + val * Y2;
+                oqx[go] = qx;
+                oqy[go] = qy;
             }
             i2 = i1; i1 = i0;
             s2 = s1; sq2 = sq1; v2 = v1;
@@ -674,6 +679,8 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             X2 = X1; Y2 = Y1; W2 = W1;
         }
     }
+    // a strip handed back is redone (and checked) by the general launch
+    if (valid) raise_health(a, bad);
     dtmin_out = dtmin;
     return valid;
 }

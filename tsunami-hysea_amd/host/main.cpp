@@ -1,12 +1,12 @@
 // main.cpp -- TsunamiHySEA <dataFile>: the reference's command line
 // (src/ShallowWater.cxx:29-166) on the MI355X library.
 //
-// Reads the reference's parameter file and NetCDF grid, runs the time loop on one
-// gfx950 GPU through the C-ABI (include/thysea.h), and writes <prefix>.nc (when
+// Reads the reference's parameter file and NetCDF grid and, as the reference's main,
+// calls shallowWater (include/thysea_sw.h, libthysea_sw.so), which runs the time loop
+// on one gfx950 GPU through the C-ABI (include/thysea.h) and writes <prefix>.nc (when
 // NetCDF saving is enabled) and <prefix>_ts.nc (when a points file is given).
 #include <sys/stat.h>
 
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,21 +19,13 @@
 #include "ncclassic.h"
 #include "outputs.h"
 #include "problem.h"
+#include "../../include/thysea_sw.h"
 
 using namespace thyhost;
 
 namespace {
 
 const std::string kVersion = "1.3.0";
-
-struct RunState {
-    Problem *p;
-    thy_ctx *ctx;
-    GridOutput *grid;
-    std::vector<float> buf[3];
-    bool io_error = false;
-    std::string err;
-};
 
 void usage(const char *prog) {
     std::cerr << "/**********************************************************************" << std::endl;
@@ -76,38 +68,6 @@ void usage(const char *prog) {
     std::cerr << "  Maximum allowed velocity of water" << std::endl;
     std::cerr << "  L (typical length)" << std::endl << "  H (typical depth)" << std::endl;
     std::cerr << "  Threshold for arrival times (m) (epsilon h if not specified)" << std::endl;
-}
-
-void on_okada(void *user, int k) {
-    (void)user;
-    fprintf(stdout, "Applying Okada with fault %d\n", k + 1);
-}
-
-void on_step(void *user, int64_t it, double dt, double t) {
-    RunState *rs = (RunState *)user;
-    fprintf(stdout, "Iteration %3lld, deltaT = %e sec, ", (long long)it, dt * rs->p->T);
-    fprintf(stdout, "Time = %g sec\n", t * rs->p->T);
-}
-
-void on_snapshot(void *user, int index, double t, thy_ctx *ctx) {
-    RunState *rs = (RunState *)user;
-    if (!rs->grid || rs->io_error) return;
-    const int fields[3] = {THY_ETA, THY_UX, THY_UY};
-    for (int k = 0; k < 3; k++) {
-        if (thy_get_field(ctx, fields[k], rs->buf[k].data()) != THY_OK) {
-            rs->io_error = true;
-            rs->err = thy_last_error(ctx);
-            return;
-        }
-    }
-    if (!rs->grid->write_snapshot(index, t * rs->p->T, rs->buf[0].data(), rs->buf[1].data(), rs->buf[2].data(),
-                                  &rs->err))
-        rs->io_error = true;
-}
-
-int env_int(const char *name, int def) {
-    const char *v = getenv(name);
-    return v ? atoi(v) : def;
 }
 
 }  // namespace
@@ -206,108 +166,37 @@ int main(int argc, char *argv[]) {
     print_problem(p, kVersion);
     std::cout << std::scientific;
 
-    // THYSEA_PRECISION=1 selects the FP64-everywhere path (precision study)
-    const int precision = env_int("THYSEA_PRECISION", 0);
-    thy_grid g = p.grid();
-    thy_params prm = p.params(precision, 1);
-    thy_ctx *ctx = nullptr;
-    int err = thy_create(&g, &prm, &ctx);
-    if (err == THY_OK) err = thy_upload_state(ctx, p.state1.data(), p.state2.data());
+    // the reference's call into its GPU part (src/ShallowWater.cxx:146-151), with its
+    // arrays built as cargarDatosProblema leaves them
+    const int nf = (int)p.faults.size();
+    std::vector<double> fv[10];
+    for (const thy_fault &f : p.faults) {
+        const double v[10] = {f.lon_c, f.lat_c, f.depth_c, f.length, f.width, f.strike, f.dip, f.rake, f.slip, f.time};
+        for (int k = 0; k < 10; k++) fv[k].push_back(v[k]);
+    }
+    for (auto &a : fv) a.resize(std::max<size_t>(1, a.size()));
+    std::vector<thy_double2> ayc(p.ny);
+    for (int j = 0; j < p.ny; j++) ayc[j] = thy_double2{p.area[j], p.cosphi[j]};
+    std::vector<double> ancho(p.ancho), alto(p.alto), lon(p.lon), lat(p.lat);
+    thy_submalla nivel{ayc.data(), ancho.data(), alto.data(), lon.data(), lat.data()};
+    std::vector<int> pos(p.ppos);
+    std::vector<double> plon(p.plon), plat(p.plat);
+    std::string name = p.name, version = kVersion;
+    double tiempo = 0.0;
+    const int err = shallowWater(1, p.okada_flag, nf, p.crop_flag, p.crop_value, fv[0].data(), fv[1].data(),
+                                 fv[2].data(), fv[3].data(), fv[4].data(), fv[5].data(), fv[6].data(), fv[7].data(),
+                                 fv[8].data(), fv[9].data(), reinterpret_cast<thy_double2 *>(p.state1.data()),
+                                 reinterpret_cast<thy_double2 *>(p.state2.data()), nivel, p.read_points,
+                                 (int)pos.size(), pos.data(), plon.data(), plat.data(), p.nx, p.ny,
+                                 (int64_t)p.nx * p.ny, p.hmin, &name[0], p.prefix, p.borde_sup, p.borde_inf,
+                                 p.borde_izq, p.borde_der, p.sp_sup, p.sp_inf, p.sp_izq, p.sp_der, p.t_end, p.t_save,
+                                 p.t_series, p.cfl, p.mf0, p.vmax, p.eps_h, p.hpos, p.cvis, p.dif_at, p.L, p.H, p.Q,
+                                 p.T, &version[0], &tiempo);
     if (err != THY_OK) {
         if (err == THY_ERR_DEVICE_MEMORY) std::cerr << "Error: Not enough GPU memory" << std::endl;
         else if (err == THY_ERR_HOST_MEMORY) std::cerr << "Error: Not enough CPU memory" << std::endl;
-        else std::cerr << "Error: " << (ctx ? thy_last_error(ctx) : "device setup failed") << std::endl;
-        thy_destroy(ctx);
         return EXIT_FAILURE;
     }
-    // strip height of the row-sweep kernel chosen by measurement (THYSEA_AUTOTUNE=0 keeps
-    // the default); the tuner leaves the state as it was
-    if (precision == 0 && env_int("THYSEA_AUTOTUNE", 1)) {
-        int rows = 0;
-        double ms = 0.0;
-        if (thy_autotune(ctx, 3, &rows, nullptr, &ms) != THY_OK)
-            std::cerr << "Warning: autotune failed: " << thy_last_error(ctx) << std::endl;
-    }
-    RunState rs;
-    rs.p = &p;
-    rs.ctx = ctx;
-    GridOutput grid;
-    SeriesOutput series;
-    std::string ioerr;
-    const size_t n = (size_t)p.nx * p.ny;
-    if (p.t_save >= 0.0) {
-        if (!grid.create(p.prefix + ".nc", p, kVersion, &ioerr)) std::cerr << ioerr << std::endl;
-        rs.grid = &grid;
-        for (auto &b : rs.buf) b.resize(n);
-    } else {
-        rs.grid = nullptr;
-    }
-    if (p.read_points == 1 && !series.create(p.prefix + "_ts.nc", p, kVersion, &ioerr)) std::cerr << ioerr << std::endl;
-
-    // initial dt from the pre-deformation state with the first truncation
-    // (src/GPU/ShallowWater.cu:378-392); thy_run computes the same value
-    double dt0 = 0.0;
-    if (thy_initial_dt(ctx, &dt0) == THY_OK) {
-        if (!p.faults.empty()) {
-            const double T = p.T;
-            if (p.faults[0].time < 2.220446049250313e-16) {
-                size_t i = 1;
-                while (i < p.faults.size() && !(std::fabs(p.faults[i].time - p.faults[0].time) > 2.220446049250313e-16)) i++;
-                if (i < p.faults.size() && dt0 * T > p.faults[i].time) dt0 = p.faults[i].time / T;
-            } else if (dt0 * T > p.faults[0].time) {
-                dt0 = p.faults[0].time / T;
-            }
-        }
-        fprintf(stdout, "Initial deltaT = %e sec\n", dt0 * p.T);
-    }
-
-    thy_run_config cfg{};
-    cfg.t_end = p.t_end;
-    cfg.save_interval = p.t_save;
-    cfg.series_interval = p.read_points == 1 ? p.t_series : -1.0;
-    cfg.n_faults = (int)p.faults.size();
-    cfg.faults = p.faults.data();
-    cfg.crop_flag = p.crop_flag;
-    cfg.crop_value = p.crop_value;
-    cfg.n_gauges = (int)p.ppos.size();
-    cfg.gauge_pos = p.ppos.data();
-    cfg.max_steps = 0;
-    cfg.steps_per_sync = env_int("THYSEA_STEPS_PER_SYNC", 0);
-    cfg.on_snapshot = on_snapshot;
-    cfg.on_okada = on_okada;
-    cfg.on_step = env_int("THYSEA_QUIET", 0) ? nullptr : on_step;
-    cfg.user = &rs;
-    thy_run_result res{};
-    const auto t0 = std::chrono::high_resolution_clock::now();
-    err = thy_run(ctx, &cfg, &res);
-    const auto t1 = std::chrono::high_resolution_clock::now();
-    if (err != THY_OK) {
-        std::cerr << "Error: " << thy_last_error(ctx) << std::endl;
-        thy_destroy(ctx);
-        return EXIT_FAILURE;
-    }
-    // final outputs (src/GPU/ShallowWater.cu:455-492)
-    if (rs.grid) {
-        std::vector<float> def(n), maxh(n), arr(n);
-        thy_get_field(ctx, THY_DEFORMED_BATHY, def.data());
-        thy_get_field(ctx, THY_MAX_HEIGHT, maxh.data());
-        thy_get_field(ctx, THY_ARRIVAL_TIMES, arr.data());
-        if (!grid.write_final(p.okada_flag == OKADA_STANDARD ? def.data() : nullptr, maxh.data(), arr.data(), &ioerr) ||
-            !grid.close(&ioerr))
-            std::cerr << ioerr << std::endl;
-    }
-    if (p.read_points == 1) {
-        const int ns = res.n_samples, ng = (int)p.ppos.size();
-        std::vector<float> tm(std::max(1, ns)), e((size_t)std::max(1, ns * ng)), u(e.size()), v(e.size());
-        std::vector<float> mn(std::max(1, ng)), mx(std::max(1, ng)), bat(std::max(1, ng));
-        thy_get_gauges(ctx, tm.data(), e.data(), u.data(), v.data(), mn.data(), mx.data(), bat.data());
-        if (!series.write_samples(ns, tm.data(), e.data(), u.data(), v.data(), &ioerr) ||
-            !series.write_final(bat.data(), mn.data(), mx.data(), &ioerr) || !series.close(&ioerr))
-            std::cerr << ioerr << std::endl;
-    }
-    if (rs.io_error) std::cerr << rs.err << std::endl;
-    const double secs = std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count() * 1e-6;
-    std::cout << std::endl << "Runtime: " << secs << " sec" << std::endl;
-    thy_destroy(ctx);
+    std::cout << std::endl << "Runtime: " << tiempo << " sec" << std::endl;
     return EXIT_SUCCESS;
 }

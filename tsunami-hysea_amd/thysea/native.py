@@ -64,6 +64,45 @@ class RunResult(C.Structure):
                 ("t_final", C.c_double), ("dt_initial", C.c_double), ("seconds", C.c_double)]
 
 
+class Request(C.Structure):
+    """thy_request (include/thysea_serve.h)."""
+    _fields_ = [("arrival_s", C.c_double), ("grid", Grid), ("params", Params), ("state1", _dp), ("state2", _dp),
+                ("run", RunConfig)]
+
+
+class RequestResult(C.Structure):
+    """thy_request_result (include/thysea_serve.h)."""
+    _fields_ = [("status", C.c_int), ("admitted_s", C.c_double), ("finished_s", C.c_double), ("steps", C.c_int64),
+                ("n_samples", C.c_int), ("max_height", _fp), ("arrival_times", _fp), ("g_time", _fp),
+                ("g_eta", _fp), ("g_u", _fp), ("g_v", _fp), ("gauge_capacity", C.c_int)]
+
+
+class ServeOptions(C.Structure):
+    """thy_serve_options (include/thysea_serve.h)."""
+    _fields_ = [("capacity", C.c_int), ("hbm_budget", C.c_size_t), ("steps_per_sync", C.c_int),
+                ("workers", C.c_int), ("burst", C.c_int)]
+
+
+SERVE_LIB_PATH = os.path.join(_PKG, "libthysea_serve.so")
+_SERVE = None
+
+
+def serve_lib():
+    """libthysea_serve.so: the C++ request loop over the batched C-ABI."""
+    global _SERVE
+    if _SERVE is None:
+        lib()  # libthysea.so first, from this package
+        if not os.path.exists(SERVE_LIB_PATH):
+            raise ThyError(f"{SERVE_LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(SERVE_LIB_PATH)
+        L.thy_serve_batched.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(ServeOptions),
+                                        C.POINTER(RequestResult), _dp]
+        L.thy_serve_sequential.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(ServeOptions),
+                                           C.POINTER(RequestResult)]
+        _SERVE = L
+    return _SERVE
+
+
 class ThyError(RuntimeError):
     def __init__(self, msg, code=None):
         super().__init__(msg)
@@ -176,7 +215,7 @@ class Solver:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {lib().thy_last_error(self.h).decode()}")
+            raise ThyError(f"{what}: {STATUS.get(rc, rc)}: {lib().thy_last_error(self.h).decode()}", rc)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -309,7 +348,7 @@ class Solver:
         return out
 
 
-SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
+SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED, SLOT_FAILED = 0, 1, 2, 3
 
 
 class Batch:
@@ -383,6 +422,9 @@ class Batch:
     def result(self, slot):
         """Max height, arrival times and gauges of a finished scenario."""
         st, r = self.status(slot)
+        if st == SLOT_FAILED:
+            msg = lib().thy_last_error(self.solvers[slot].h).decode()
+            raise ThyError(f"slot {slot} failed: {msg}", STATUS_CODES["NONFINITE"])
         if st != SLOT_FINISHED:
             raise ThyError(f"slot {slot} is not finished")
         s = self.solvers[slot]

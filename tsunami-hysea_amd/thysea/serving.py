@@ -1,157 +1,126 @@
-"""Request loop over the batched C-ABI (thy_batch_*): scenarios are admitted as they
-arrive and as the HBM budget allows, their state stays resident, and every running
-scenario advances in each captured step sequence; finished ones hand back max
-height, arrival times and gauges and free their slot.  A plain loop running the same
-requests one at a time through thy_run is the comparison.
+"""Python driver of the request loop (include/thysea_serve.h, libthysea_serve.so).
+
+The loop itself is C++ (tsunami-hysea_amd/host/serve.cpp): scenarios are admitted as
+they arrive and as the HBM budget allows, their state stays resident, and every
+running scenario advances in each captured step sequence; finished ones hand back max
+height, arrival times and gauges and free their slot.  The comparison serves the same
+requests one at a time through thy_run.  This module only turns Problem objects into
+request descriptions and reads the results back.
 
 Time-to-result of a request = from its arrival to its max-height, arrival-time and
 gauge arrays in host memory (input generation happens before the clock starts, as
 reading input files would)."""
-import time
-from concurrent.futures import ThreadPoolExecutor
+import ctypes as C
 
 import numpy as np
 
 from . import native
 
 
-def _result(r):
-    return {"max_height": r["max_height"], "arrival_times": r["arrival_times"], "g_eta": r["g_eta"],
-            "steps": r["steps"]}
+class _Requests:
+    """thy_request array for ``problems`` with the arrays it points to kept alive, and
+    result buffers for the requests in ``keep``."""
+
+    def __init__(self, problems, arrivals, keep, steps_per_sync, solver_kw):
+        n = len(problems)
+        self.n = n
+        self.req = (native.Request * max(1, n))()
+        self.out = (native.RequestResult * max(1, n))()
+        self.alive = []
+        self.bufs = {}
+        for i, pr in enumerate(problems):
+            q = self.req[i]
+            q.arrival_s = float(arrivals[i])
+            geo = [np.ascontiguousarray(a, np.float64) for a in (pr.lon, pr.lat, pr.area, pr.cosphi, pr.ancho, pr.alto)]
+            q.grid = native.Grid(pr.nx, pr.ny, *[a.ctypes.data_as(native._dp) for a in geo])
+            q.params = native.params_of(pr, **(solver_kw or {}))
+            s1 = np.empty((pr.nx * pr.ny, 2))
+            s2 = np.empty((pr.nx * pr.ny, 2))
+            s1[:, 0], s1[:, 1] = np.ravel(pr.h), np.ravel(pr.H)
+            s2[:, 0], s2[:, 1] = np.ravel(pr.qx), np.ravel(pr.qy)
+            q.state1 = s1.ctypes.data_as(native._dp)
+            q.state2 = s2.ctypes.data_as(native._dp)
+            faults = sorted(pr.faults, key=lambda f: f["time"])
+            farr = (native.Fault * max(1, len(faults)))(*[native.fault_of(f) for f in faults])
+            gpos = np.ascontiguousarray(pr.gauge_pos if pr.gauge_pos is not None else np.zeros(0), np.int32)
+            c = q.run
+            c.t_end, c.save_interval, c.series_interval = pr.t_end, -1.0, pr.series_interval
+            c.n_faults, c.faults = len(faults), farr
+            c.crop_flag, c.crop_value = pr.crop_flag, pr.crop_value
+            c.n_gauges, c.gauge_pos = len(gpos), gpos.ctypes.data_as(native._ip)
+            c.max_steps, c.steps_per_sync = 0, int(steps_per_sync)
+            self.alive.append((geo, s1, s2, farr, gpos))
+            if i in keep:
+                ng = len(gpos)
+                cap = int(pr.t_end / pr.series_interval) + 8 if (pr.series_interval > 0 and ng) else 0
+                b = {"max_height": np.empty((pr.ny, pr.nx), np.float32),
+                     "arrival_times": np.empty((pr.ny, pr.nx), np.float32),
+                     "g_time": np.zeros(max(1, cap), np.float32),
+                     "g_eta": np.zeros((max(1, cap), max(1, ng)), np.float32),
+                     "g_u": np.zeros((max(1, cap), max(1, ng)), np.float32),
+                     "g_v": np.zeros((max(1, cap), max(1, ng)), np.float32)}
+                o = self.out[i]
+                for k in b:
+                    setattr(o, k, b[k].ctypes.data_as(native._fp))
+                o.gauge_capacity = cap
+                self.bufs[i] = (b, ng)
+
+    def records(self, arrivals):
+        rec = []
+        for i in range(self.n):
+            o = self.out[i]
+            if o.status != 0:
+                raise native.ThyError(f"request {i}: {native.STATUS.get(o.status, o.status)}", o.status)
+            rec.append(dict(index=i, arrival=float(arrivals[i]), admitted=o.admitted_s, finished=o.finished_s,
+                            steps=int(o.steps)))
+        return rec
+
+    def results(self):
+        out = {}
+        for i, (b, ng) in self.bufs.items():
+            ns = min(self.out[i].n_samples, self.out[i].gauge_capacity)
+            out[i] = {"max_height": b["max_height"], "arrival_times": b["arrival_times"],
+                      "steps": int(self.out[i].steps), "n_samples": ns, "g_time": b["g_time"][:ns],
+                      "g_eta": b["g_eta"][:ns, :ng], "g_u": b["g_u"][:ns, :ng], "g_v": b["g_v"][:ns, :ng]}
+        return out
+
+
+def _options(capacity=512, hbm_budget=0, steps_per_sync=48, workers=8, burst=8):
+    o = native.ServeOptions()
+    o.capacity, o.hbm_budget, o.steps_per_sync = int(capacity), int(hbm_budget), int(steps_per_sync)
+    o.workers, o.burst = int(workers), int(burst)
+    return o
 
 
 def run_batched(problems, arrivals=None, capacity=512, hbm_budget=0, steps_per_sync=48, keep=None, burst=8,
                 workers=8, solver_kw=None):
-    """Serve ``problems`` (arrival times in seconds, default all at 0) through one
-    Batch.  Returns per-request records and, for indices in ``keep``, the results.
-
-    The host work of a request runs while the GPU steps the batch: each step sequence
-    is launched asynchronously and the loop admits arrived requests and retires
-    finished ones until it has completed (at most ``burst`` admissions while the GPU is
-    idle, so that stepping starts early).  The work that touches one context only
-    (creating and uploading it, preparing its run with the faults due at t = 0;
-    reading out its max height, arrival times and gauges and releasing it) runs on
-    ``workers`` threads; the batch itself is driven from this thread.  ``solver_kw`` goes to every native.Solver (strip heights, for one)."""
+    """Serve ``problems`` (arrival times in seconds, default all at 0) through one batch
+    with the C++ request loop (thy_serve_batched).  Returns per-request records
+    (arrival, admitted, finished, steps), the results of the requests in ``keep`` (max
+    height, arrival times, gauges) and the GPU seconds of the batch's step launches.
+    ``solver_kw`` sets the kernel parameters of every request (strip heights, for one)."""
     n = len(problems)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
-    order = list(np.argsort(arrivals, kind="stable"))
-    keep = set(keep or ())
-    rec = [dict(arrival=float(arrivals[i]), admitted=None, finished=None, steps=0) for i in range(n)]
-    results = {}
-    b = native.Batch(capacity, hbm_budget=hbm_budget, steps_per_sync=steps_per_sync)
-    pool = ThreadPoolExecutor(max_workers=max(1, workers))
-    live = {}        # slot -> (request index, solver)
-    pending = []     # finished slots not yet retired
-    prepared = {}    # request index -> future of its created and uploaded solver
-    readouts = []    # (request index, future of its results)
-    t0 = time.perf_counter()
-    state = {"nxt": 0, "nsub": 0, "done": 0}
+    rq = _Requests(problems, arrivals, set(keep or ()), steps_per_sync, solver_kw)
+    gpu = C.c_double()
+    opt = _options(capacity, hbm_budget, steps_per_sync, workers, burst)
+    rc = native.serve_lib().thy_serve_batched(rq.req, n, C.byref(opt), rq.out, C.byref(gpu))
+    if rc != 0:
+        raise native.ThyError(f"thy_serve_batched: {native.STATUS.get(rc, rc)}", rc)
+    return rq.records(arrivals), rq.results(), gpu.value
 
-    def now():
-        return time.perf_counter() - t0
 
-    def prepare(i):
-        s = native.Solver(problems[i], **(solver_kw or {}))
-        s.prepare_batch(steps_per_sync=steps_per_sync)
-        return s
-
-    def readout(s, n_samples, steps):
-        out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"), "steps": steps}
-        out.update(s.gauges(n_samples))
-        s.close()
-        return out, now()
-
-    def submit_prepares():
-        # keep up to two requests per worker in preparation, in arrival order
-        while (state["nsub"] < n and arrivals[order[state["nsub"]]] <= now()
-               and state["nsub"] - state["nxt"] < 2 * max(1, workers)):
-            i = order[state["nsub"]]
-            prepared[i] = pool.submit(prepare, i)
-            state["nsub"] += 1
-
-    def admit(block):
-        """Admit the next request: True, False (budget or slots full), None (not prepared yet)."""
-        i = order[state["nxt"]]
-        f = prepared.get(i)
-        if f is None or (not block and not f.done()):
-            return None
-        s = f.result()
-        slot = b.add(s, steps_per_sync=steps_per_sync)
-        if slot is None:
-            if not live:
-                s.close()
-                raise native.ThyError(f"request {i} does not fit the batch's HBM budget")
-            return False
-        del prepared[i]
-        live[slot] = (i, s)
-        rec[i]["admitted"] = now()
-        state["nxt"] += 1
-        return True
-
-    def retire(slot):
-        i, s = live.pop(slot)
-        r = b.status(slot)[1]
-        b.remove(slot)
-        readouts.append((i, pool.submit(readout, s, r["n_samples"], r["steps"])))
-
-    def collect(block=False):
-        rest = []
-        for i, f in readouts:
-            if block or f.done():
-                out, t_done = f.result()
-                rec[i]["finished"] = t_done
-                rec[i]["steps"] = int(out["steps"])
-                if i in keep:
-                    results[i] = _result(out)
-                state["done"] += 1
-            else:
-                rest.append((i, f))
-        readouts[:] = rest
-
-    in_flight = False
-    try:
-        while state["done"] < n:
-            submit_prepares()
-            k = 0
-            while True:
-                collect()
-                if in_flight:
-                    if b.ready():
-                        break
-                elif k >= burst:
-                    break
-                if pending:
-                    retire(pending.pop(0))
-                    continue
-                if state["nxt"] < n and arrivals[order[state["nxt"]]] <= now():
-                    ok = admit(block=not in_flight)
-                    if ok:
-                        k += 1
-                        submit_prepares()
-                        continue
-                    if ok is False:
-                        break
-                    submit_prepares()   # it arrived after the last submission
-                    time.sleep(0.0002)  # or it is still being prepared
-                    continue
-                break
-            if in_flight:
-                b.wait()
-                in_flight = False
-                pending += [slot for slot in live if slot not in pending and b.status(slot)[0] == native.SLOT_FINISHED]
-            if any(b.status(slot)[0] == native.SLOT_RUNNING for slot in live):
-                b.launch()
-                in_flight = True
-            elif not pending:
-                if readouts:
-                    collect(block=True)
-                elif state["nxt"] < n:
-                    time.sleep(max(0.0, min(0.01, arrivals[order[state["nxt"]]] - now())))
-    finally:
-        pool.shutdown(wait=True)
-    gpu = b.info()["gpu_seconds"]
-    b.close()
-    return rec, results, gpu
+def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None, solver_kw=None):
+    """The same requests one at a time through thy_run, first come first served
+    (thy_serve_sequential)."""
+    n = len(problems)
+    arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
+    rq = _Requests(problems, arrivals, set(keep or ()), steps_per_sync, solver_kw)
+    opt = _options(steps_per_sync=steps_per_sync)
+    rc = native.serve_lib().thy_serve_sequential(rq.req, n, C.byref(opt), rq.out)
+    if rc != 0:
+        raise native.ThyError(f"thy_serve_sequential: {native.STATUS.get(rc, rc)}", rc)
+    return rq.records(arrivals), rq.results()
 
 
 def tune_batch(problems, candidates=(24, 32, 40, 48), n=32, launches=2, steps_per_sync=48):
@@ -189,31 +158,6 @@ def tune_batch(problems, candidates=(24, 32, 40, 48), n=32, launches=2, steps_pe
     if not cost:
         raise native.ThyError("no candidate strip height advanced a scenario")
     return min(cost, key=cost.get), cost
-
-
-def run_sequential(problems, arrivals=None, steps_per_sync=48, keep=None, solver_kw=None):
-    """The same requests one at a time through thy_run (first come, first served)."""
-    n = len(problems)
-    arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, np.float64)
-    keep = set(keep or ())
-    rec, results = [], {}
-    t0 = time.perf_counter()
-    for i in np.argsort(arrivals, kind="stable"):
-        now = time.perf_counter() - t0
-        if arrivals[i] > now:
-            time.sleep(arrivals[i] - now)
-        s = native.Solver(problems[i], **(solver_kw or {}))
-        start = time.perf_counter() - t0
-        r = s.run(steps_per_sync=steps_per_sync)
-        out = {"max_height": s.field("max_height"), "arrival_times": s.field("arrival_times"),
-               "g_eta": r["g_eta"], "steps": r["steps"]}
-        s.close()
-        rec.append(dict(index=int(i), arrival=float(arrivals[i]), admitted=start,
-                        finished=time.perf_counter() - t0, steps=int(r["steps"])))
-        if i in keep:
-            results[int(i)] = out
-    rec.sort(key=lambda x: x["index"])
-    return rec, results
 
 
 def latency_stats(rec):
