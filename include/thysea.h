@@ -133,7 +133,7 @@ const char *thy_last_error(const thy_ctx *ctx);
 
 /* Upload the reference's state arrays: state1 = (h, H) pairs and state2 = (qx, qy)
    pairs, interleaved as the double2 arrays datosVolumenesNivel_1/_2
-   (src/GPU/ShallowWater.cu:299-300). */
+   (src/GPU/ShallowWater.cu:299-300).  THY_ERR_NONFINITE if any value is NaN or Inf. */
 int thy_upload_state(thy_ctx *ctx, const double *state1, const double *state2);
 /* Inverse of thy_upload_state: (h, H) and (qx, qy) as doubles. */
 int thy_download_state(thy_ctx *ctx, double *state1, double *state2);
@@ -164,7 +164,10 @@ int thy_run(thy_ctx *ctx, const thy_run_config *cfg, thy_run_result *res);
 
 /* Continue the schedule of the last thy_run for up to n_steps more steps (or until
    its end time).  No reference counterpart: the reference loop cannot be resumed;
-   this lets a caller time an exact number of steps after a warm-up. */
+   this lets a caller time an exact number of steps after a warm-up.  n_steps = 0 takes
+   no step but captures and uploads the step sequences the next call replays, so that
+   call starts at once.  Any step count replays without idle launches: a sequence is
+   kept for each launch slot it can start at (launch n uses control and dt slot n mod 3). */
 int thy_run_more(thy_ctx *ctx, int64_t n_steps, thy_run_result *res);
 
 /* Output field conversion to float (src/GPU/netcdf_kernel.cu), [ny*nx] row-major. */

@@ -34,6 +34,9 @@ typedef struct thy_request {
     thy_params params;       /* FP32 row-sweep kernel for batched runs */
     const double *state1;    /* (h, H) pairs, as thy_upload_state */
     const double *state2;    /* (qx, qy) pairs */
+    /* the same state as four arrays [ny*nx], used when state1 is NULL (the loop pairs
+       them on a worker thread; requests may share them) */
+    const double *h, *H, *qx, *qy;
     thy_run_config run;      /* t_end, faults, gauges, crop; no snapshots, no callbacks */
 } thy_request;
 

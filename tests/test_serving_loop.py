@@ -60,6 +60,8 @@ class Reqs:
             lon = np.array([float(i), float(i) + 1.0])
             geo = np.zeros(ny + 1)
             q.grid = native.Grid(2, ny, lon.ctypes.data_as(native._dp), *[geo.ctypes.data_as(native._dp)] * 5)
+            st = np.zeros(2 * ny)
+            q.h = q.H = q.qx = q.qy = st.ctypes.data_as(native._dp)
             q.run.t_end = float(steps)
             gpos = np.zeros(2, np.int32)
             q.run.n_gauges, q.run.gauge_pos = 2, gpos.ctypes.data_as(native._ip)
@@ -70,7 +72,7 @@ class Reqs:
             for k, v in b.items():
                 setattr(self.out[i], k, v.ctypes.data_as(native._fp))
             self.out[i].gauge_capacity = ns
-            self.keep.append((lon, geo, gpos))
+            self.keep.append((lon, geo, gpos, st))
             self.res.append(b)
 
     def check(self):

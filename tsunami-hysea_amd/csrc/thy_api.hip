@@ -7,7 +7,8 @@
 // Device layout (structure of arrays, row pitch a multiple of 64 elements):
 //   e[2], qx[2], qy[2]   ping-pong state (eta relative to sea level, discharges)
 //   D                    still-water depth (changes only at Okada time)
-//   emax, tarr, eini     metrics (tarr/eini only when arrival times are tracked)
+//   emax, arr            metrics (arr, when arrival times are tracked: eta_ini until a
+//                        cell arrives, then its arrival time; see Arrival<T>)
 //   row[ny], fy[ny+1]    per-row / per-face-row geometry constants
 //   ctrl[3], dtmin[3]    device control block and dt slots
 #include <hip/hip_runtime.h>
@@ -34,7 +35,7 @@ struct thy_ctx {
     size_t esz = 4;
     std::vector<double> lon, lat, area, cosphi, ancho, alto;
     void *e[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
-    void *D = nullptr, *emax = nullptr, *tarr = nullptr, *eini = nullptr;
+    void *D = nullptr, *emax = nullptr, *arr = nullptr;
     void *row = nullptr, *fy = nullptr;
     double *area_d = nullptr;
     Ctrl *ctrl = nullptr;
@@ -88,6 +89,11 @@ struct thy_ctx {
     double last_ms = 0.0;
     std::vector<float> g_time, g_eta, g_u, g_v;
     int nsamples = 0;
+    bool gauges_stale = false;   // the device holds samples not yet copied to g_*
+    // host copy of the control block in launch slot `slot`, when the host knows it
+    // (written by the host, or collected after a replay); saves a synchronous read
+    Ctrl ctrl_h{};
+    bool ctrl_h_valid = false;
     // schedule of the current run (kept so that thy_run_more can continue it)
     std::vector<thy_fault> faults;
     std::vector<double> ftimes;
@@ -212,8 +218,7 @@ StepArgs<T> make_args(thy_ctx *c) {
     }
     a.D = (const T *)c->D;
     a.emax = (T *)c->emax;
-    a.tarr = (T *)c->tarr;
-    a.eini = (const T *)c->eini;
+    a.arr = (T *)c->arr;
     a.row = (const RowGeo<T> *)c->row;
     a.fy = (const FaceGeoY<T> *)c->fy;
     a.ctrl = c->ctrl;
@@ -282,6 +287,7 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
 }
 
 hipError_t launch_step(thy_ctx *c) {
+    c->ctrl_h_valid = false;  // the launch publishes a control block the host has not seen
     hipError_t e = c->prec ? launch_step_t<double>(c, c->slot) : launch_step_t<float>(c, c->slot);
     c->slot = (c->slot + 1) % 3;
     return e;
@@ -598,6 +604,9 @@ int upload_t(thy_ctx *ctx, const double *s1, const double *s2) {
         for (int i = 0; i < nx; i++) {
             const size_t c = r + i, o = ro + i;
             const double h = s1[2 * c], Hs = s1[2 * c + 1];
+            if (!std::isfinite(h) || !std::isfinite(Hs) || !std::isfinite(s2[2 * c]) || !std::isfinite(s2[2 * c + 1]))
+                return fail(ctx, THY_ERR_NONFINITE,
+                            "non-finite input state at cell (" + std::to_string(i) + ", " + std::to_string(j) + ")");
             const double Dd = Hs + hmin;             // unshifted depth
             D[o] = (T)Dd;
             e[o] = (h > 0.0) ? (T)(h - Dd) : -(T)Dd;  // dry: e = -D exactly, so h = 0
@@ -648,13 +657,22 @@ int download_t(thy_ctx *ctx, double *s1, double *s2) {
 }
 
 int read_ctrl(thy_ctx *ctx, Ctrl *c) {
+    if (ctx->ctrl_h_valid) {
+        *c = ctx->ctrl_h;
+        return THY_OK;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx_copy(ctx, c, &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+    ctx->ctrl_h = *c;
+    ctx->ctrl_h_valid = true;
     return THY_OK;
 }
 
 int write_ctrl(thy_ctx *ctx, const Ctrl &c) {
+    ctx->ctrl_h_valid = false;
     HIPCHK(ctx_copy(ctx, &ctx->ctrl[ctx->slot], &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+    ctx->ctrl_h = c;
+    ctx->ctrl_h_valid = true;
     return THY_OK;
 }
 
@@ -695,7 +713,7 @@ int apply_okada_t(thy_ctx *ctx, const thy_fault *f, int crop_flag, double crop_v
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_okada_apply<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, ctx->okada_def, ctx->nx,
                        ctx->ny, ctx->pitch, crop_flag, crop_value, &ctx->scratch_u[1], (T *)ctx->e[ctx->cur],
-                       (T *)nullptr, (T *)ctx->D, (T *)ctx->eini);
+                       (T *)nullptr, (T *)ctx->D, (T *)ctx->arr);
     HIPCHK(hipGetLastError());
     if (def_out) {
         HIPCHK(hipMemcpyAsync(def_out, ctx->okada_def, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -709,7 +727,7 @@ int init_metrics_t(thy_ctx *ctx) {
     const int64_t n = (int64_t)ctx->nx * ctx->ny;
     hipLaunchKernelGGL((k_init_metrics<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, (const T *)ctx->e[ctx->cur],
                        (const T *)ctx->D, ctx->nx, ctx->ny, ctx->pitch, (T)ctx->prm.eps_h, (T *)ctx->emax,
-                       (T *)ctx->eini, (T *)ctx->tarr);
+                       (T *)ctx->arr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return THY_OK;
@@ -734,13 +752,13 @@ int initial_dt_t(thy_ctx *ctx, double *dt) {
 template <typename T>
 int get_field_t(thy_ctx *ctx, int field, float *out) {
     const int64_t n = (int64_t)ctx->nx * ctx->ny;
-    if (field == THY_ARRIVAL_TIMES && !ctx->tarr) {
+    if (field == THY_ARRIVAL_TIMES && !ctx->arr) {
         for (int64_t i = 0; i < n; i++) out[i] = -1.0f;
         return THY_OK;
     }
     hipLaunchKernelGGL((k_output<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream, field, (const T *)ctx->e[ctx->cur],
                        (const T *)ctx->qx[ctx->cur], (const T *)ctx->qy[ctx->cur], (const T *)ctx->D,
-                       (const T *)ctx->emax, (const T *)ctx->tarr, ctx->nx, ctx->ny, ctx->pitch, (T)ctx->prm.eps_h,
+                       (const T *)ctx->emax, (const T *)ctx->arr, ctx->nx, ctx->ny, ctx->pitch, (T)ctx->prm.eps_h,
                        ctx->prm.H, ctx->prm.Q, ctx->prm.T, ctx->outbuf);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, ctx->outbuf, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -753,8 +771,7 @@ int update_metrics_t(thy_ctx *ctx, double t) {
     const int64_t n = (int64_t)ctx->nx * ctx->ny;
     hipLaunchKernelGGL((k_update_metrics<T>), dim3(grid_1d(n)), dim3(256), 0, ctx->stream,
                        (const T *)ctx->e[ctx->cur], (const T *)ctx->D, ctx->nx, ctx->ny, ctx->pitch,
-                       (T)ctx->prm.eps_h, (T)ctx->prm.dif_at, (T)t, (T *)ctx->emax, (const T *)ctx->eini,
-                       (T *)ctx->tarr);
+                       (T)ctx->prm.eps_h, (T)ctx->prm.dif_at, (T)t, (T *)ctx->emax, (T *)ctx->arr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return THY_OK;
@@ -852,18 +869,16 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
     if ((r = build_graphs(ctx, K)) != THY_OK) return r;
     Ctrl last;
     if ((r = read_ctrl(ctx, &last)) != THY_OK) return r;
-    {
-        unsigned int aux[2];
-        HIPCHK(ctx_copy(ctx, &aux[0], &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost));
-        HIPCHK(ctx_copy(ctx, &aux[1], ctx->health, 4, hipMemcpyDeviceToHost));
-        if (aux[1]) last.nonfinite = 1;
-        if (must_stop(last, ctx->ftimes.data())) {  // stopped before any step
-            if ((r = finalise_on_host(ctx, last, aux[0])) != THY_OK) return r;
-            *out = last;
-            if (ms) *ms = 0.0;
-            return THY_OK;
-        }
+    if (must_stop(last, ctx->ftimes.data())) {  // stopped before any step
+        unsigned int raw = 0;
+        if (!last.dt_valid)
+            HIPCHK(ctx_copy(ctx, &raw, &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost));
+        if ((r = finalise_on_host(ctx, last, raw)) != THY_OK) return r;
+        *out = last;
+        if (ms) *ms = 0.0;
+        return THY_OK;
     }
+    // (a health flag raised before this call stops the first launch on the device)
     double total = 0.0, dt_mean = 0.0;
     int head = 0, queued = 0;
     auto enqueue = [&]() -> hipError_t {
@@ -922,7 +937,12 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         if (c.dt_valid || host_stop) {
             while (queued > 0) HIPCHK(collect(c));  // a queued replay idled: same control block
             ctx->cur = c.cur;
-            if (host_stop && (r = finalise_on_host(ctx, c, raw)) != THY_OK) return r;
+            if (host_stop) {
+                if ((r = finalise_on_host(ctx, c, raw)) != THY_OK) return r;
+            } else {
+                ctx->ctrl_h = c;  // the control block the stopping launch left in this slot
+                ctx->ctrl_h_valid = true;
+            }
             *out = c;
             if (ms) *ms = total;
             return THY_OK;
@@ -996,7 +1016,7 @@ size_t thy_footprint(int nx, int ny, const thy_params *p) {
     const size_t esz = (p && p->precision) ? 8 : 4;
     const size_t pitch = ((size_t)nx + 63) / 64 * 64;
     const size_t n = pitch * ny;
-    size_t fields = 7 + 1 + ((p && p->track_arrival) ? 2 : 0);  // e,qx,qy x2, D, emax, (tarr, eini)
+    size_t fields = 7 + 1 + ((p && p->track_arrival) ? 1 : 0);  // e,qx,qy x2, D, emax, (arr)
     size_t bytes = fields * n * esz;
     bytes += (size_t)nx * ny * (8 + 4);                 // okada scratch (double) + output staging (float)
     bytes += (size_t)(ny + 1) * 64 * 2;                 // geometry
@@ -1023,6 +1043,9 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     ctx->alto.assign(g->alto, g->alto + g->ny);
     int r = pick_tile(ctx);
     if (r != THY_OK) return r;
+    // the row sweeps address every array with 32-bit byte offsets; a grid whose arrays
+    // are larger runs the tile kernel
+    if (ctx->kernel != 1 && (size_t)ctx->pitch * ctx->ny * ctx->esz > (size_t)INT32_MAX) ctx->kernel = 1;
     HIPCHK(hipSetDevice(0));
     HIPCHK(Pool::get().get_streams(ctx->stream, ctx->side));
     {
@@ -1049,10 +1072,7 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     }
     HIPCHK(dev_alloc(&ctx->D, b));
     HIPCHK(dev_alloc(&ctx->emax, b));
-    if (p->track_arrival) {
-        HIPCHK(dev_alloc(&ctx->tarr, b));
-        HIPCHK(dev_alloc(&ctx->eini, b));
-    }
+    if (p->track_arrival) HIPCHK(dev_alloc(&ctx->arr, b));
     HIPCHK(dev_alloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
     HIPCHK(dev_alloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
     HIPCHK(dev_alloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
@@ -1108,7 +1128,7 @@ void thy_destroy(thy_ctx *ctx) {
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     drop_graphs(ctx);
     void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
-                    ctx->tarr, ctx->eini, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
+                    ctx->arr, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
                     ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list, ctx->health};
     for (void *q : ptrs)
@@ -1314,10 +1334,17 @@ int thy_last_timing(thy_ctx *ctx, double *ms) {
     return THY_OK;
 }
 
+static int collect_gauges(thy_ctx *ctx);
+
 int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v, float *eta_min, float *eta_max,
                    float *bathy) {
     RelaxedCapture relaxed;
     if (!ctx) return THY_ERR_INVALID;
+    if (ctx->gauges_stale) {
+        const int r = collect_gauges(ctx);
+        if (r != THY_OK) return r;
+        ctx->gauges_stale = false;
+    }
     const int ns = ctx->nsamples, ng = ctx->ng;
     if (time) std::copy(ctx->g_time.begin(), ctx->g_time.begin() + ns, time);
     if (eta) std::copy(ctx->g_eta.begin(), ctx->g_eta.begin() + (size_t)ns * ng, eta);
@@ -1364,7 +1391,6 @@ static int drain_log(thy_ctx *ctx, int64_t upto) {
     return THY_OK;
 }
 
-static int collect_gauges(thy_ctx *ctx);
 
 static int run_loop(thy_ctx *ctx, thy_run_result *res) {
     int r;
@@ -1411,7 +1437,8 @@ static int run_loop(thy_ctx *ctx, thy_run_result *res) {
     res->n_snapshots = ctx->nsnap;
     res->seconds = total_ms * 1e-3;
     ctx->last_ms = total_ms;
-    return collect_gauges(ctx);
+    ctx->gauges_stale = true;  // copied to the host when thy_get_gauges asks for them
+    return THY_OK;
 }
 
 // gauge samples of the finished run into host memory (read by thy_get_gauges)
@@ -1835,6 +1862,7 @@ int thy_batch_add(thy_batch *b, thy_ctx *ctx, const thy_run_config *cfg, int *sl
         res = ctx->prep_res;
     }
     ctx->batch_ready = false;
+    ctx->ctrl_h_valid = false;  // the batch's launches advance its control blocks
     const StepArgs<float> a = make_args<float>(ctx);
     // ordered after a launch in flight, which does not read this free slot
     b->h_args[slot] = a;
@@ -1920,7 +1948,7 @@ int thy_batch_wait(thy_batch *b, int *n_running) {
             // stopped: finished, or a fault is due (the only other stop a batched run has)
             if (!(cb.t < cb.t_end) || (cb.max_steps > 0 && cb.steps >= cb.max_steps)) {
                 c->nsamples = std::min(cb.nsamples, c->gcap);
-                if ((r = collect_gauges(c)) != THY_OK) return bfail(b, r, thy_last_error(c));
+                c->gauges_stale = true;  // read by thy_get_gauges, on the caller's thread
                 thy_run_result &res = b->res[i];
                 res.steps = cb.steps;
                 res.t_final = cb.t;

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) k_okada(OkadaFault f, int nx, int ny, dou
 template <typename T>
 __global__ void __launch_bounds__(256) k_okada_apply(const double *def, int nx, int ny, int P, int crop_flag,
                                                      double crop_value, const unsigned int *absmax, T *e0, T *e1,
-                                                     T *D, T *eini) {
+                                                     T *D, T *arr) {
     double thr = -1.0;
     if (crop_flag == 1) thr = crop_value * (double)__uint_as_float(*absmax);
     else if (crop_flag == 2) thr = crop_value;
@@ -223,7 +223,8 @@ __global__ void __launch_bounds__(256) k_okada_apply(const double *def, int nx, 
         D[o] = (T)((double)D[o] - u);
         e0[o] = (T)((double)e0[o] + u);
         if (e1) e1[o] = (T)((double)e1[o] + u);
-        if (eini) eini[o] = (T)((double)eini[o] + u);
+        // eta_ini += U_Z (a cell that has arrived keeps its arrival time)
+        if (arr && !Arrival<T>::arrived(arr[o])) arr[o] = (T)((double)arr[o] + u);
     }
 }
 
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256) k_okada_apply(const double *def, int nx, 
 // src/GPU/ShallowWater.cu:305-309)
 template <typename T>
 __global__ void __launch_bounds__(256) k_init_metrics(const T *e, const T *D, int nx, int ny, int P, T eps_h,
-                                                      T *emax, T *eini, T *tarr) {
+                                                      T *emax, T *arr) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < (int64_t)nx * ny;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(c / nx), i = (int)(c - (int64_t)j * nx);
@@ -239,29 +240,28 @@ __global__ void __launch_bounds__(256) k_init_metrics(const T *e, const T *D, in
         const T h = e[o] + D[o];
         const T v = (h < eps_h) ? T(-1e30) : e[o];
         emax[o] = v;
-        if (eini) eini[o] = v;
-        if (tarr) tarr[o] = T(-1);
+        if (arr) arr[o] = v;  // eta_ini, by the same kernel as eta_max (src/GPU/ShallowWater.cu:305-308)
     }
 }
 
 // metric update outside the step kernel (src/GPU/Metrics.cu:23-42,57-77)
 template <typename T>
 __global__ void __launch_bounds__(256) k_update_metrics(const T *e, const T *D, int nx, int ny, int P, T eps_h,
-                                                        T dif_at, T t, T *emax, const T *eini, T *tarr) {
+                                                        T dif_at, T t, T *emax, T *arr) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < (int64_t)nx * ny;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(c / nx), i = (int)(c - (int64_t)j * nx);
         const int64_t o = (int64_t)j * P + i;
         const T h = e[o] + D[o];
         if (e[o] > emax[o] && h > eps_h) emax[o] = e[o];
-        if (tarr && tarr[o] < T(0) && tabs(e[o] - eini[o]) > dif_at) tarr[o] = t;
+        if (arr && !Arrival<T>::arrived(arr[o]) && tabs(e[o] - arr[o]) > dif_at) arr[o] = Arrival<T>::enc(t);
     }
 }
 
 // output fields (src/GPU/netcdf_kernel.cu:6-126) into a dense [ny][nx] float array
 template <typename T>
 __global__ void __launch_bounds__(256) k_output(int which, const T *e, const T *qx, const T *qy, const T *D,
-                                                const T *emax, const T *tarr, int nx, int ny, int P, T eps_h,
+                                                const T *emax, const T *arr, int nx, int ny, int P, T eps_h,
                                                 double Href, double Q, double Tscale, float *out) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < (int64_t)nx * ny;
          c += (int64_t)gridDim.x * blockDim.x) {
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(256) k_output(int which, const T *e, const T *
         } else if (which == 4) {
             v = ((double)emax[o] < -1e20) ? -9999.0 : (double)emax[o] * Href;
         } else {
-            v = tarr ? (((double)tarr[o] < 0.0) ? -1.0 : (double)tarr[o] * Tscale) : -1.0;
+            v = (arr && Arrival<T>::arrived(arr[o])) ? Arrival<T>::dec(arr[o]) * Tscale : -1.0;
         }
         out[c] = (float)v;
     }

@@ -45,6 +45,21 @@ struct alignas(16) Ctrl {
     int pad;
 };
 
+// Arrival tracking keeps one value per cell (src/GPU/Metrics.cu:44-77 keeps two): the
+// cell's eta_ini while the wave has not arrived, and once it has, the arrival time t
+// stored as t * 2^100 (2^40 for t = 0), exactly (a power-of-two scale).  The arrival
+// test reads one array instead of two.  A value at or above 2^39 means "arrived";
+// eta_ini values are far below that (the largest initial |eta| is the dry-cell marker
+// -1e30, on the other side of zero).
+template <typename T> struct Arrival {
+    static constexpr float kArrived = 0x1p39f;
+    static constexpr float kZero = 0x1p40f;  // arrived at t = 0
+    static constexpr float kScale = 0x1p100f;
+    __host__ __device__ static T enc(const T t) { return t > T(0) ? t * T(kScale) : T(kZero); }
+    __host__ __device__ static bool arrived(const T p) { return p >= T(kArrived); }
+    __host__ __device__ static double dec(const T p) { return p == T(kZero) ? 0.0 : (double)p * 0x1p-100; }
+};
+
 template <typename T> struct RowGeo {
     T A, invA, gx, valx;  // area, 1/area, 1/cos(phi), (2*alto/cos)/(2*area)
     T lenx, dxs, dys, pad; // alto (x-face length), ancho and alto for sponge distances
@@ -76,8 +91,8 @@ template <typename T> struct StepArgs {
     T Href, Q_over_H;
     T *e[2], *qx[2], *qy[2];
     const T *D;
-    T *emax, *tarr;
-    const T *eini;
+    T *emax;
+    T *arr;                  // arrival tracking: eta_ini until the cell arrives, then the arrival time (Arrival<T>)
     const RowGeo<T> *row;
     const FaceGeoY<T> *fy;
     Ctrl *ctrl;
@@ -310,8 +325,8 @@ __device__ __forceinline__ void metrics_cell(const StepArgs<T> &a, const int64_t
     const T em = a.emax[go];
     if (eo > em && ho > a.eps_h) a.emax[go] = eo;
     if (a.track_arrival) {
-        const T ta = a.tarr[go];
-        if (ta < T(0) && tabs(eo - a.eini[go]) > a.dif_at) a.tarr[go] = tnow;
+        const T p = a.arr[go];
+        if (!Arrival<T>::arrived(p) && tabs(eo - p) > a.dif_at) a.arr[go] = Arrival<T>::enc(tnow);
     }
 }
 

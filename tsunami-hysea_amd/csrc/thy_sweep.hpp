@@ -129,6 +129,40 @@ template <typename T> struct RowCell {
     T e, D, qx, qy;
 };
 
+// The row sweeps address each array through a buffer resource over the whole array:
+// the lane's column is the vector offset and the row the scalar offset, both 32-bit
+// (every array is under 2^31 bytes; thy_create keeps larger grids off these kernels).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t whole_array(const void *p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Vector offset of a buffer store that must not happen: at or beyond the resource's
+// record count, so the hardware drops it.  A store whose lanes may all be masked is
+// issued this way instead of under a branch, so that every iteration of a row loop
+// issues the same number of memory instructions and the compiler can wait for a row's
+// loads without waiting for the stores issued after them (one counter covers both).
+constexpr int kDropOffset = (int)0x80000000;
+
+template <typename T>
+__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff, bool on);
+template <>
+__device__ __forceinline__ void buf_store<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff, bool on) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, on ? voff : kDropOffset, soff, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff, bool on) {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, on ? voff : kDropOffset, soff, 0);
+}
+
 template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T qt) {
     SC<T> c;
     c.e = e;
@@ -190,29 +224,32 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
 
         auto clamp_row = [&](int r) { return r < 0 ? 0 : (r >= ny ? ny - 1 : r); };
         auto clamp_face = [&](int g) { return g < 0 ? 0 : (g > ny ? ny : g); };
-        // loads go through a wavefront-uniform row base plus the lane's column, and are
-        // never predicated (rows are clamped onto the grid), so the compiler can keep
-        // the prefetches in flight across iterations
-        const unsigned ucx = (unsigned)cx;
+        // buffer loads: the row (clamped onto the grid) in the scalar offset, the lane's
+        // column in the vector offset; never predicated, so the compiler can keep the
+        // prefetches in flight across iterations
+        const int bytes = P * ny * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t re = whole_array(ge, bytes), rD = whole_array(gD, bytes);
+        const __amdgpu_buffer_rsrc_t rqx = whole_array(gqx, bytes), rqy = whole_array(gqy, bytes);
+        const __amdgpu_buffer_rsrc_t roe = whole_array(oe, bytes), roqx = whole_array(oqx, bytes);
+        const __amdgpu_buffer_rsrc_t roqy = whole_array(oqy, bytes);
+        const int vcx = cx * (int)sizeof(T);
         auto load_row = [&](int r) {
-            const int64_t o = (int64_t)clamp_row(r) * P;
+            const int so = clamp_row(r) * P * (int)sizeof(T);
             RowCell<T> c;
-            c.e = (ge + o)[ucx];
-            c.D = (gD + o)[ucx];
-            c.qx = (gqx + o)[ucx];
-            c.qy = (gqy + o)[ucx];
+            c.e = buf_load<T>(re, vcx, so);
+            c.D = buf_load<T>(rD, vcx, so);
+            c.qx = buf_load<T>(rqx, vcx, so);
+            c.qy = buf_load<T>(rqy, vcx, so);
             return c;
         };
-        // metrics inputs: without arrival tracking the arrival loads read emax instead
-        const T *__restrict__ gmax = a.emax;
-        const T *__restrict__ gtar = a.track_arrival ? a.tarr : a.emax;
-        const T *__restrict__ gini = a.track_arrival ? a.eini : a.emax;
-        const unsigned gxm = (unsigned)(gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx));
-        auto load_metrics = [&](int jn, T &m, T &t, T &i) {
-            const int64_t o = (int64_t)(jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P;
-            m = (gmax + o)[gxm];
-            t = (gtar + o)[gxm];
-            i = (gini + o)[gxm];
+        // metrics inputs: without arrival tracking the arrival load reads emax instead
+        const __amdgpu_buffer_rsrc_t rmax = whole_array(a.emax, bytes);
+        const __amdgpu_buffer_rsrc_t rarr = whole_array(a.track_arrival ? a.arr : a.emax, bytes);
+        const int vgx = (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)) * (int)sizeof(T);
+        auto load_metrics = [&](int jn, T &m, T &t) {
+            const int so = (jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P * (int)sizeof(T);
+            m = buf_load<T>(rmax, vgx, so);
+            t = buf_load<T>(rarr, vgx, so);
         };
 
         // the segment touches the first-order band of the x borders: no x face of it
@@ -234,16 +271,16 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         bool din3 = false, din2 = false, din1 = false;
         bool ds5 = false, ds4 = false, ds3 = false;
         // One pipeline iteration for row r.  buf holds row r on entry and receives row
-        // r+1; (pm, pt, pi) hold the metrics inputs of row r-4 on entry and receive those
-        // of row r-3: every load has one iteration of work behind it.  (Keeping loads two
+        // r+1; (pm, pt) hold the metrics inputs of row r-4 on entry and receive those of
+        // row r-3: every load has one iteration of work behind it.  (Keeping loads two
         // iterations ahead needs a two-way unrolled loop, whose register use costs more
         // occupancy than the extra latency cover gains.)
-        auto iteration = [&](const int r, RowCell<T> &buf, T &pm, T &pt, T &pi) {
+        auto iteration = [&](const int r, RowCell<T> &buf, T &pm, T &pt) {
             i0 = buf;
             buf = load_row(r + 1);
             const int ju = r - 4;  // row updated in this iteration
-            const T cmax = pm, ctar = pt, cini = pi;
-            load_metrics(ju + 1, pm, pt, pi);
+            const T cmax = pm, carr = pt;
+            load_metrics(ju + 1, pm, pt);
             const T h0 = i0.e + i0.D;
             const bool din0 = __all(h0 > hpos) != 0;
 
@@ -362,25 +399,37 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 W3 += f.w;
             }
             // ---- cell update of row r-4 (src/GPU/Volumen_kernel.cu:160-221) ----
-            if (ju >= y0 && ju < y1 && out_col) {
-                const RowGeo<T> rg = uniform_load(a.row + ju);
-                const int64_t go = (int64_t)ju * P + gx;  // gx = gxm on output lanes
-                const T D = i4.D, eo = i4.e;
-                // metrics from the state before the step (src/GPU/Metrics.cu:23-42,57-77)
-                if (eo > cmax && eo + D > a.eps_h) a.emax[go] = eo;
-                if (a.track_arrival && ctar < T(0) && tabs(eo - cini) > a.dif_at) a.tarr[go] = tnow;
-                const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
-                dtmin = fminf(dtmin, (float)dtc);
-                const T val = dt * rg.invA;
-                T qx = i4.qx + val * X4;
-                T qy = i4.qy + val * Y4;
-                if (any_sponge) sponge_q(a, gx, ju, qx, qy);
-                const T hs = s4 + D;
-                if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
-                bad = bad || bad_cell(s4, qx, qy);
-                oe[go] = s4;
-                oqx[go] = qx;
-                oqy[go] = qy;
+            // The arithmetic runs for output rows only; the five stores are issued in
+            // every iteration and dropped where they do not apply.
+            {
+                const bool row_on = ju >= y0 && ju < y1;
+                const bool on = row_on && out_col;
+                const int soff = (row_on ? ju : y0) * P * (int)sizeof(T);
+                const int voff = on ? gx * (int)sizeof(T) : kDropOffset;
+                const T eo = i4.e;
+                bool max_on = false, arr_on = false;
+                T qx = T(0), qy = T(0);
+                if (row_on) {
+                    const RowGeo<T> rg = uniform_load(a.row + ju);
+                    const T D = i4.D;
+                    // metrics from the state before the step (src/GPU/Metrics.cu:23-42,57-77)
+                    max_on = on && eo > cmax && eo + D > a.eps_h;
+                    arr_on = on && a.track_arrival && !Arrival<T>::arrived(carr) && tabs(eo - carr) > a.dif_at;
+                    const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
+                    if (out_col) dtmin = fminf(dtmin, (float)dtc);
+                    const T val = dt * rg.invA;
+                    qx = i4.qx + val * X4;
+                    qy = i4.qy + val * Y4;
+                    if (any_sponge) sponge_q(a, gx, ju, qx, qy);
+                    const T hs = s4 + D;
+                    if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
+                    bad = bad || (on && bad_cell(s4, qx, qy));
+                }
+                buf_store<T>(eo, rmax, voff, soff, max_on);
+                buf_store<T>(Arrival<T>::enc(tnow), rarr, voff, soff, arr_on);
+                buf_store<T>(s4, roe, voff, soff, on);
+                buf_store<T>(qx, roqx, voff, soff, on);
+                buf_store<T>(qy, roqy, voff, soff, on);
             }
             // ---- advance the pipeline ----
             i5 = i4; i4 = i3; i3 = i2; i2 = i1; i1 = i0;
@@ -396,9 +445,12 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // rows y0-4 .. y1+3
         const int rbeg = y0 - 4, rend = y1 + 3;
         RowCell<T> buf = load_row(rbeg);
-        T pm, pt, pi;
-        load_metrics(rbeg - 4, pm, pt, pi);
-        for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt, pi);
+        T pm, pt;
+        load_metrics(rbeg - 4, pm, pt);
+        // enter the loop with nothing outstanding, so that the wait at its head only
+        // accounts for the previous iteration (see kDropOffset)
+        __builtin_amdgcn_s_waitcnt(0);
+        for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt);
     }
     raise_health(a, bad);
     return dtmin;
@@ -522,10 +574,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
 // launches evaluate the deep faces with the same functions, so the split changes no
 // result.
 // ---------------------------------------------------------------------------
-// buffer resource over a whole device array (raw, 32-bit offsets)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t whole_array(const void *p, int bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
-}
+
 
 // One wavefront's strip on the deep-water path.  The 2s faces reach one row and one
 // column, so the pipeline lags two rows: iteration r loads row r, evaluates the mass
@@ -562,8 +611,10 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         const __amdgpu_buffer_rsrc_t rqy = whole_array(c1 ? a.qy[1] : a.qy[0], bytes);
         const __amdgpu_buffer_rsrc_t rD = whole_array(a.D, bytes);
         const __amdgpu_buffer_rsrc_t rmax = whole_array(a.emax, bytes);
-        const __amdgpu_buffer_rsrc_t rtar = whole_array(a.track_arrival ? a.tarr : a.emax, bytes);
-        const __amdgpu_buffer_rsrc_t rini = whole_array(a.track_arrival ? a.eini : a.emax, bytes);
+        const __amdgpu_buffer_rsrc_t rarr = whole_array(a.track_arrival ? a.arr : a.emax, bytes);
+        const __amdgpu_buffer_rsrc_t roe = whole_array(oe, bytes);
+        const __amdgpu_buffer_rsrc_t roqx = whole_array(oqx, bytes);
+        const __amdgpu_buffer_rsrc_t roqy = whole_array(oqy, bytes);
         const int gx = x0 - SW_HALO + lane;
         const int voff = gx * (int)sizeof(T);
         auto ld = [&](const __amdgpu_buffer_rsrc_t r, const int row) {
@@ -588,17 +639,20 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         T s2 = T(0), sq2 = T(0), v2 = T(0), m1 = T(0);
         T X2 = T(0), Y2 = T(0), W2 = T(0);
         RowCell<T> buf = load_row(y0 - 3);
-        T pm = ld(rmax, y0), pt = ld(rtar, y0), pi = ld(rini, y0);
+        T pm = ld(rmax, y0), pt = ld(rarr, y0);
+        // Enter the loop with no memory instruction outstanding: the wait at the loop
+        // head then only has the previous iteration to account for (its stores may stay
+        // in flight while the next row loads), instead of a full wait every iteration.
+        __builtin_amdgcn_s_waitcnt(0);
         for (int r = y0 - 3; r <= y1 + 2; r++) {
             const RowCell<T> i0 = buf;
             buf = load_row(r < y1 + 2 ? r + 1 : r);
             const int ju = r - 2;  // row updated in this iteration
-            const T cmax = pm, ctar = pt, cini = pi;
+            const T cmax = pm, carr = pt;
             {
                 const int jn = metrics_row(ju + 1);
                 pm = ld(rmax, jn);
-                pt = ld(rtar, jn);
-                pi = ld(rini, jn);
+                pt = ld(rarr, jn);
             }
             const T h0 = i0.e + i0.D;
             valid = valid && __all(h0 > hpos);
@@ -657,21 +711,26 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 X1 -= f.pt;
                 W1 += f.w;
             }
-            // cell update of row r-2 (src/GPU/Volumen_kernel.cu:160-221; h* > hpos here)
-            if (ju >= y0 && ju < y1 && out_col) {
-                const RowGeo<T> rg = uniform_load(a.row + ju);
-                const int64_t go = (int64_t)ju * P + gx;
+            // cell update of row r-2 (src/GPU/Volumen_kernel.cu:160-221; h* > hpos here).
+            // Every iteration issues the five stores; those of halo rows and halo lanes,
+            // and metric stores that do not apply, are dropped.
+            {
+                const bool on = ju >= y0 && ju < y1 && out_col;
+                const int jr = (ju >= y0 && ju < y1) ? ju : y0;  // a row inside the arrays
+                const RowGeo<T> rg = uniform_load(a.row + jr);
+                const int soff = jr * P * (int)sizeof(T);
                 const T D = i2.D, eo = i2.e;
-                if (eo > cmax && eo + D > a.eps_h) a.emax[go] = eo;
-                if (a.track_arrival && ctar < T(0) && tabs(eo - cini) > a.dif_at) a.tarr[go] = tnow;
+                buf_store<T>(eo, rmax, voff, soff, on && eo > cmax && eo + D > a.eps_h);
+                buf_store<T>(Arrival<T>::enc(tnow), rarr, voff, soff,
+                              on && a.track_arrival && !Arrival<T>::arrived(carr) && tabs(eo - carr) > a.dif_at);
                 const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
-                dtmin = fminf(dtmin, (float)dtc);
+                dtmin = on ? fminf(dtmin, (float)dtc) : dtmin;
                 const T val = dt * rg.invA;
                 const T qx = i2.qx + val * X2, qy = i2.qy + val * Y2;
-                bad = bad || bad_cell(s2, qx, qy);
-                oe[go] = s2;
-                oqx[go] = qx;
-                oqy[go] = qy;
+                bad = bad || (on && bad_cell(s2, qx, qy));
+                buf_store<T>(s2, roe, voff, soff, on);
+                buf_store<T>(qx, roqx, voff, soff, on);
+                buf_store<T>(qy, roqy, voff, soff, on);
             }
             i2 = i1; i1 = i0;
             s2 = s1; sq2 = sq1; v2 = v1;

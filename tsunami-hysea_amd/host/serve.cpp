@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -113,8 +114,35 @@ struct Prepared {
 
 Prepared create_upload(const thy_request &q) {
     Prepared p;
+    if (!q.state1 && (!q.h || !q.H || !q.qx || !q.qy)) {
+        p.status = THY_ERR_INVALID;
+        return p;
+    }
     p.status = thy_create(&q.grid, &q.params, &p.ctx);
-    if (p.status == THY_OK) p.status = thy_upload_state(p.ctx, q.state1, q.state2);
+    if (p.status == THY_OK) {
+        if (q.state1) {
+            p.status = thy_upload_state(p.ctx, q.state1, q.state2);
+        } else {
+            // pair the four arrays as the reference's double2 state
+            const size_t n = (size_t)q.grid.nx * q.grid.ny;
+            std::vector<double> s1, s2;
+            try {
+                s1.resize(2 * n);
+                s2.resize(2 * n);
+            } catch (const std::bad_alloc &) {
+                p.status = THY_ERR_HOST_MEMORY;
+            }
+            if (p.status == THY_OK) {
+                for (size_t c = 0; c < n; c++) {
+                    s1[2 * c] = q.h[c];
+                    s1[2 * c + 1] = q.H[c];
+                    s2[2 * c] = q.qx[c];
+                    s2[2 * c + 1] = q.qy[c];
+                }
+                p.status = thy_upload_state(p.ctx, s1.data(), s2.data());
+            }
+        }
+    }
     if (p.status != THY_OK && p.ctx) {
         thy_destroy(p.ctx);
         p.ctx = nullptr;

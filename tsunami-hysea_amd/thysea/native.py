@@ -67,7 +67,7 @@ class RunResult(C.Structure):
 class Request(C.Structure):
     """thy_request (include/thysea_serve.h)."""
     _fields_ = [("arrival_s", C.c_double), ("grid", Grid), ("params", Params), ("state1", _dp), ("state2", _dp),
-                ("run", RunConfig)]
+                ("h", _dp), ("H", _dp), ("qx", _dp), ("qy", _dp), ("run", RunConfig)]
 
 
 class RequestResult(C.Structure):

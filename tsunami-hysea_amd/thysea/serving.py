@@ -34,12 +34,10 @@ class _Requests:
             geo = [np.ascontiguousarray(a, np.float64) for a in (pr.lon, pr.lat, pr.area, pr.cosphi, pr.ancho, pr.alto)]
             q.grid = native.Grid(pr.nx, pr.ny, *[a.ctypes.data_as(native._dp) for a in geo])
             q.params = native.params_of(pr, **(solver_kw or {}))
-            s1 = np.empty((pr.nx * pr.ny, 2))
-            s2 = np.empty((pr.nx * pr.ny, 2))
-            s1[:, 0], s1[:, 1] = np.ravel(pr.h), np.ravel(pr.H)
-            s2[:, 0], s2[:, 1] = np.ravel(pr.qx), np.ravel(pr.qy)
-            q.state1 = s1.ctypes.data_as(native._dp)
-            q.state2 = s2.ctypes.data_as(native._dp)
+            # the state as four arrays (shared by requests on one grid), paired by the
+            # loop's worker threads
+            soa = [np.ascontiguousarray(a, np.float64).ravel() for a in (pr.h, pr.H, pr.qx, pr.qy)]
+            q.h, q.H, q.qx, q.qy = [a.ctypes.data_as(native._dp) for a in soa]
             faults = sorted(pr.faults, key=lambda f: f["time"])
             farr = (native.Fault * max(1, len(faults)))(*[native.fault_of(f) for f in faults])
             gpos = np.ascontiguousarray(pr.gauge_pos if pr.gauge_pos is not None else np.zeros(0), np.int32)
@@ -49,7 +47,7 @@ class _Requests:
             c.crop_flag, c.crop_value = pr.crop_flag, pr.crop_value
             c.n_gauges, c.gauge_pos = len(gpos), gpos.ctypes.data_as(native._ip)
             c.max_steps, c.steps_per_sync = 0, int(steps_per_sync)
-            self.alive.append((geo, s1, s2, farr, gpos))
+            self.alive.append((geo, soa, farr, gpos))
             if i in keep:
                 ng = len(gpos)
                 cap = int(pr.t_end / pr.series_interval) + 8 if (pr.series_interval > 0 and ng) else 0
