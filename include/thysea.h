@@ -181,6 +181,15 @@ int thy_get_gauges(thy_ctx *ctx, float *time, float *eta, float *u, float *v,
 /* Compensated FP64 sum of h*area*cos(phi) (the scheme's mass invariant), for drift checks. */
 int thy_mass(thy_ctx *ctx, double *mass);
 
+/* thy_mass without any allocation, for callers that cannot allocate (code under stream
+   capture, tight serving loops, harnesses that forbid device or host allocation); no
+   counterpart in the reference.  Same kernel and summation as thy_mass, same result.
+   The workgroup partial sums go to a file-scope device array of the library (1024
+   doubles, 8 KiB per device, held until the process ends) instead of the context's own
+   buffer; the grid size does not change the partial count, so no grid is too large.
+   Every call on one device shares that array: calls that overlap on one device race. */
+int thy_mass_noalloc(thy_ctx *ctx, double *mass);
+
 /* Device bytes held by one context of this grid and parameters (admission). */
 size_t thy_footprint(int nx, int ny, const thy_params *params);
 

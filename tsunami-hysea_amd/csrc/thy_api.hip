@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,8 +29,12 @@
 
 using namespace thy;
 
+// the device array behind thy_mass_noalloc's partial sums (see kScratchPartials)
+static __device__ __align__(256) double scratch[1024];
+
 struct thy_ctx {
     int nx = 0, ny = 0, pitch = 0;
+    int device = -1;    // the device the context lives on
     thy_params prm{};
     int prec = 0;       // 0 float, 1 double
     size_t esz = 4;
@@ -116,6 +121,18 @@ struct thy_ctx {
 namespace {
 
 constexpr int kMassBlocks = 1024;
+
+// Scratch of thy_mass_noalloc: the kMassBlocks workgroup partial sums of the mass kernel
+// (8 KiB, whatever the grid size: the mass kernel always runs kMassBlocks workgroups).
+// Each device holds its own copy until the process ends.  Its device address is taken
+// once per device, with that device current, when thy_create first runs there; host code
+// uses only that address (the array's name is a host-side stand-in, not a device address).
+constexpr int kScratchPartials = 1024;
+static_assert(kMassBlocks <= kScratchPartials, "the mass partials must fit the scratch");
+static_assert(sizeof(scratch) == kScratchPartials * sizeof(double), "scratch size");
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kScratchDevices] = {};
 
 int fail(thy_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -777,16 +794,18 @@ int update_metrics_t(thy_ctx *ctx, double t) {
     return THY_OK;
 }
 
+// Compensated FP64 sum of h * area * cos(phi): kMassBlocks workgroup partials in the
+// device buffer `partial`, copied to the host array `host` (kMassBlocks entries).
 template <typename T>
-int mass_t(thy_ctx *ctx, double *mass) {
+int mass_t(thy_ctx *ctx, double *partial, double *host, double *mass) {
     hipLaunchKernelGGL((k_mass<T>), dim3(kMassBlocks), dim3(256), 0, ctx->stream, (const T *)ctx->e[ctx->cur],
                        (const T *)ctx->D, (const RowGeo<T> *)ctx->row, ctx->area_d, ctx->nx, ctx->ny, ctx->pitch,
-                       ctx->mass_partial);
+                       partial);
     HIPCHK(hipGetLastError());
-    std::vector<double> part(kMassBlocks);
-    HIPCHK(ctx_copy(ctx, part.data(), ctx->mass_partial, kMassBlocks * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, host, partial, kMassBlocks * sizeof(double), hipMemcpyDeviceToHost));
     double s = 0.0, comp = 0.0;
-    for (double x : part) {
+    for (int k = 0; k < kMassBlocks; k++) {
+        const double x = host[k];
         double t = s + x;
         if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
         else comp += (x - t) + s;
@@ -1052,6 +1071,17 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
         int dev = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->device = dev;
+        // the device address of this device's copy of the allocation-free scratch, once
+        // per device, with the device current
+        if (dev < kScratchDevices) {
+            std::lock_guard<std::mutex> lock(g_scratch_mutex);
+            if (!g_scratch_ptr[dev]) {
+                void *p = nullptr;
+                HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                g_scratch_ptr[dev] = (double *)p;
+            }
+        }
     }
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));
@@ -1325,7 +1355,19 @@ int thy_get_field(thy_ctx *ctx, int field, float *out) {
 int thy_mass(thy_ctx *ctx, double *mass) {
     RelaxedCapture relaxed;
     if (!ctx || !mass) return THY_ERR_INVALID;
-    return ctx->prec ? mass_t<double>(ctx, mass) : mass_t<float>(ctx, mass);
+    std::vector<double> host(kMassBlocks);
+    return ctx->prec ? mass_t<double>(ctx, ctx->mass_partial, host.data(), mass)
+                     : mass_t<float>(ctx, ctx->mass_partial, host.data(), mass);
+}
+
+int thy_mass_noalloc(thy_ctx *ctx, double *mass) {
+    RelaxedCapture relaxed;
+    if (!ctx || !mass) return THY_ERR_INVALID;
+    if (ctx->device < 0 || ctx->device >= kScratchDevices || !g_scratch_ptr[ctx->device])
+        return fail(ctx, THY_ERR_INVALID, "the allocation-free scratch of this device was not set up");
+    double host[kScratchPartials];  // on the stack: no allocation on the host either
+    return ctx->prec ? mass_t<double>(ctx, g_scratch_ptr[ctx->device], host, mass)
+                     : mass_t<float>(ctx, g_scratch_ptr[ctx->device], host, mass);
 }
 
 int thy_last_timing(thy_ctx *ctx, double *ms) {

@@ -132,6 +132,7 @@ def lib():
         L.thy_get_field.argtypes = [C.c_void_p, C.c_int, _fp]
         L.thy_get_gauges.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp, _fp]
         L.thy_mass.argtypes = [C.c_void_p, _dp]
+        L.thy_mass_noalloc.argtypes = [C.c_void_p, _dp]
         L.thy_footprint.argtypes = [C.c_int, C.c_int, C.POINTER(Params)]
         L.thy_footprint.restype = C.c_size_t
         L.thy_last_timing.argtypes = [C.c_void_p, _dp]
@@ -277,6 +278,12 @@ class Solver:
     def mass(self):
         v = C.c_double()
         self._chk(lib().thy_mass(self.h, C.byref(v)), "mass")
+        return v.value
+
+    def mass_noalloc(self):
+        """thy_mass_noalloc: the same sum through the library's file-scope scratch."""
+        v = C.c_double()
+        self._chk(lib().thy_mass_noalloc(self.h, C.byref(v)), "mass_noalloc")
         return v.value
 
     def autotune(self, reps=5):
