@@ -617,6 +617,11 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         const __amdgpu_buffer_rsrc_t roqy = whole_array(oqy, bytes);
         const int gx = x0 - SW_HALO + lane;
         const int voff = gx * (int)sizeof(T);
+        // byte addresses of the left and right neighbour lanes for ds_bpermute (the same
+        // moves as from_left / from_right, computed once per strip)
+        const int adl = (lane > 0 ? lane - 1 : lane) << 2, adr = (lane < 63 ? lane + 1 : lane) << 2;
+        auto left = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adl, __float_as_int(v))); };
+        auto right = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(v))); };
         auto ld = [&](const __amdgpu_buffer_rsrc_t r, const int row) {
             return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, row * P * (int)sizeof(T), 0));
         };
@@ -640,17 +645,17 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         T X2 = T(0), Y2 = T(0), W2 = T(0);
         RowCell<T> buf = load_row(y0 - 3);
         T pm = ld(rmax, y0), pt = ld(rarr, y0);
-        // Enter the loop with no memory instruction outstanding: the wait at the loop
-        // head then only has the previous iteration to account for (its stores may stay
-        // in flight while the next row loads), instead of a full wait every iteration.
-        __builtin_amdgcn_s_waitcnt(0);
-        for (int r = y0 - 3; r <= y1 + 2; r++) {
+        // One row of the pipeline.  In the steady state (rows y0+2 .. y1) every stage
+        // applies, so the stage conditions drop out at compile time; the other rows keep
+        // them.  Both forms evaluate the same expressions.
+        auto row = [&](const int r, auto steady_tag) {
+            constexpr bool kS = decltype(steady_tag)::value;
             const RowCell<T> i0 = buf;
-            buf = load_row(r < y1 + 2 ? r + 1 : r);
+            buf = load_row((kS || r < y1 + 2) ? r + 1 : r);
             const int ju = r - 2;  // row updated in this iteration
             const T cmax = pm, carr = pt;
             {
-                const int jn = metrics_row(ju + 1);
+                const int jn = kS ? ju + 1 : metrics_row(ju + 1);
                 pm = ld(rmax, jn);
                 pt = ld(rarr, jn);
             }
@@ -658,13 +663,13 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             valid = valid && __all(h0 > hpos);
             // mass x-faces of row r (h* rows y0-2 .. y1+1)
             T m0 = T(0);
-            if (r >= y0 - 2 && r <= y1 + 1) {
+            if (kS || (r >= y0 - 2 && r <= y1 + 1)) {
                 const RowGeo<T> rg = uniform_load(a.row + r);
-                const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
-                m0 = F - from_right(F);
+                const T F = fast_mass_x(left(i0.qx), left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
+                m0 = F - right(F);
             }
             // mass y-face g = r between rows r-1 and r (faces y0-2 .. y1+2)
-            if (r >= y0 - 2) {
+            if (kS || r >= y0 - 2) {
                 const FaceGeoY<T> fg = uniform_load(a.fy + r);
                 const T F = fast_mass_y(i1.qy, i1.e + i1.D, h0, fg, dt);
                 m1 -= F * fg.ic0;
@@ -673,8 +678,8 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             // h* of row r-1 (rows y0-2 .. y1+1)
             T s1 = T(0), hs1 = T(0), sq1 = T(0), u1 = T(0), v1 = T(0);
             const int j1 = r - 1;
-            const RowGeo<T> rg1 = uniform_load(a.row + (j1 < y0 - 2 ? y0 - 2 : j1));
-            if (j1 >= y0 - 2 && j1 <= y1 + 1) {
+            const RowGeo<T> rg1 = uniform_load(a.row + ((kS || j1 >= y0 - 2) ? j1 : y0 - 2));
+            if (kS || (j1 >= y0 - 2 && j1 <= y1 + 1)) {
                 T es = i1.e + dt * rg1.invA * m1;
                 if (!(es + i1.D > T(0))) es = -i1.D;
                 s1 = es;
@@ -687,18 +692,18 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             }
             // momentum x-faces of row r-1 (rows y0 .. y1-1)
             T X1 = T(0), Y1 = T(0), W1 = T(0);
-            if (j1 >= y0 && j1 < y1) {
+            if (kS || (j1 >= y0 && j1 < y1)) {
                 DC<T> A, B;
-                A.h = from_left(hs1); A.e = from_left(s1); A.D = from_left(i1.D);
-                A.qn = from_left(i1.qx); A.qt = from_left(i1.qy); A.s = from_left(sq1); A.un = from_left(u1);
+                A.h = left(hs1); A.e = left(s1); A.D = left(i1.D);
+                A.qn = left(i1.qx); A.qt = left(i1.qy); A.s = left(sq1); A.un = left(u1);
                 B.h = hs1; B.e = s1; B.D = i1.D; B.qn = i1.qx; B.qt = i1.qy; B.s = sq1; B.un = u1;
                 const MF<T> f = fast_mom_x(A, B, rg1.lenx, rg1.gx, rg1.valx, dt, cd2s);
-                X1 = -f.pn - from_right(f.mn);
-                Y1 = -f.pt - from_right(f.mt);
-                W1 = f.w + from_right(f.w);
+                X1 = -f.pn - right(f.mn);
+                Y1 = -f.pt - right(f.mt);
+                W1 = f.w + right(f.w);
             }
             // momentum y-face g = r-1 between rows r-2 and r-1 (faces y0 .. y1)
-            if (j1 >= y0 && j1 <= y1) {
+            if (kS || (j1 >= y0 && j1 <= y1)) {
                 const FaceGeoY<T> fg = uniform_load(a.fy + j1);
                 DC<T> A, B;
                 A.h = s2 + i2.D; A.e = s2; A.D = i2.D; A.qn = i2.qy; A.qt = i2.qx; A.s = sq2; A.un = v2;
@@ -715,8 +720,9 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             // Every iteration issues the five stores; those of halo rows and halo lanes,
             // and metric stores that do not apply, are dropped.
             {
-                const bool on = ju >= y0 && ju < y1 && out_col;
-                const int jr = (ju >= y0 && ju < y1) ? ju : y0;  // a row inside the arrays
+                const bool in_rows = kS || (ju >= y0 && ju < y1);
+                const bool on = in_rows && out_col;
+                const int jr = in_rows ? ju : y0;  // a row inside the arrays
                 const RowGeo<T> rg = uniform_load(a.row + jr);
                 const int soff = jr * P * (int)sizeof(T);
                 const T D = i2.D, eo = i2.e;
@@ -736,7 +742,21 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             s2 = s1; sq2 = sq1; v2 = v1;
             m1 = m0;
             X2 = X1; Y2 = Y1; W2 = W1;
-        }
+        };
+        using Edge = std::integral_constant<bool, false>;
+        using Steady = std::integral_constant<bool, true>;
+        // Enter the loops with no memory instruction outstanding: the wait before a row
+        // then only has the previous rows to account for (their stores may stay in flight
+        // while the next row loads), instead of a full wait every row.
+        __builtin_amdgcn_s_waitcnt(0);
+        // rows y0-3 .. y1+2: the halo rows at either end with the stage conditions, the
+        // steady state without them
+        const int pe = min(y0 + 1, y1 + 2);
+        int r = y0 - 3;
+        for (; r <= pe; r++) row(r, Edge{});
+#pragma unroll 1
+        for (; r <= y1; r++) row(r, Steady{});
+        for (; r <= y1 + 2; r++) row(r, Edge{});
     }
     // a strip handed back is redone (and checked) by the general launch
     if (valid) raise_health(a, bad);
