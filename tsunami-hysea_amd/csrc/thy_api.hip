@@ -41,6 +41,8 @@ struct thy_ctx {
     std::vector<double> lon, lat, area, cosphi, ancho, alto;
     void *e[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
     void *D = nullptr, *emax = nullptr, *arr = nullptr;
+    void *state = nullptr;   // one allocation holding e, qx, qy (x2), D, emax and arr
+    size_t state_bytes = 0;
     void *row = nullptr, *fy = nullptr;
     double *area_d = nullptr;
     Ctrl *ctrl = nullptr;
@@ -236,6 +238,8 @@ StepArgs<T> make_args(thy_ctx *c) {
     a.D = (const T *)c->D;
     a.emax = (T *)c->emax;
     a.arr = (T *)c->arr;
+    a.block = (const char *)c->state;
+    a.block_bytes = (int)std::min(c->state_bytes, (size_t)INT32_MAX);  // kernel 0 only when it fits
     a.row = (const RowGeo<T> *)c->row;
     a.fy = (const FaceGeoY<T> *)c->fy;
     a.ctrl = c->ctrl;
@@ -448,7 +452,7 @@ int build_order(thy_ctx *ctx, const T *D) {
     // (no clamping, no first-order band), of the sponges, and of coast
     // the deep-water path addresses each array through a buffer resource with a 32-bit
     // byte range
-    const bool split = c->kernel == 0 && c->prec == 0 && (size_t)P * ny * sizeof(float) <= (size_t)INT32_MAX;
+    const bool split = c->kernel == 0 && c->prec == 0;  // (kernel 0: the state block fits 32-bit offsets)
     const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
     // strips of the split launches: rows shifted up by yoff (the block order of the
     // launch without a split keeps unshifted tiles)
@@ -1031,12 +1035,15 @@ int thy_probe(void) {
     return 2;
 }
 
+// arrays of the state block: e, qx, qy (two buffers each), D, emax and, with arrival
+// tracking, arr
+static int state_arrays(const thy_params *p) { return 8 + ((p && p->track_arrival) ? 1 : 0); }
+
 size_t thy_footprint(int nx, int ny, const thy_params *p) {
     const size_t esz = (p && p->precision) ? 8 : 4;
     const size_t pitch = ((size_t)nx + 63) / 64 * 64;
     const size_t n = pitch * ny;
-    size_t fields = 7 + 1 + ((p && p->track_arrival) ? 1 : 0);  // e,qx,qy x2, D, emax, (arr)
-    size_t bytes = fields * n * esz;
+    size_t bytes = (size_t)state_arrays(p) * n * esz;
     bytes += (size_t)nx * ny * (8 + 4);                 // okada scratch (double) + output staging (float)
     bytes += (size_t)(ny + 1) * 64 * 2;                 // geometry
     return bytes;
@@ -1062,9 +1069,11 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     ctx->alto.assign(g->alto, g->alto + g->ny);
     int r = pick_tile(ctx);
     if (r != THY_OK) return r;
-    // the row sweeps address every array with 32-bit byte offsets; a grid whose arrays
-    // are larger runs the tile kernel
-    if (ctx->kernel != 1 && (size_t)ctx->pitch * ctx->ny * ctx->esz > (size_t)INT32_MAX) ctx->kernel = 1;
+    // the row sweeps address every state array through one buffer resource over the
+    // state block, with 32-bit byte offsets; a grid whose block is larger runs the tile
+    // kernel
+    ctx->state_bytes = (size_t)state_arrays(p) * ctx->pitch * ctx->ny * ctx->esz;
+    if (ctx->kernel != 1 && ctx->state_bytes > (size_t)INT32_MAX) ctx->kernel = 1;
     HIPCHK(hipSetDevice(0));
     HIPCHK(Pool::get().get_streams(ctx->stream, ctx->side));
     {
@@ -1095,14 +1104,18 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     HIPCHK(Pool::get().host_alloc((void **)&ctx->h_snap, 2 * sizeof(Ctrl)));
     HIPCHK(Pool::get().host_alloc((void **)&ctx->h_aux, 4 * sizeof(unsigned int)));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
-    for (int k = 0; k < 2; k++) {
-        HIPCHK(dev_alloc(&ctx->e[k], b));
-        HIPCHK(dev_alloc(&ctx->qx[k], b));
-        HIPCHK(dev_alloc(&ctx->qy[k], b));
+    HIPCHK(dev_alloc(&ctx->state, ctx->state_bytes));
+    {
+        char *blk = (char *)ctx->state;  // arrays of b bytes (whole 256-B rows) in a fixed order
+        for (int k = 0; k < 2; k++) {
+            ctx->e[k] = blk + (0 + k) * b;
+            ctx->qx[k] = blk + (2 + k) * b;
+            ctx->qy[k] = blk + (4 + k) * b;
+        }
+        ctx->D = blk + 6 * b;
+        ctx->emax = blk + 7 * b;
+        if (p->track_arrival) ctx->arr = blk + 8 * b;
     }
-    HIPCHK(dev_alloc(&ctx->D, b));
-    HIPCHK(dev_alloc(&ctx->emax, b));
-    if (p->track_arrival) HIPCHK(dev_alloc(&ctx->arr, b));
     HIPCHK(dev_alloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
     HIPCHK(dev_alloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
     HIPCHK(dev_alloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
@@ -1157,8 +1170,7 @@ void thy_destroy(thy_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     drop_graphs(ctx);
-    void *ptrs[] = {ctx->e[0], ctx->e[1], ctx->qx[0], ctx->qx[1], ctx->qy[0], ctx->qy[1], ctx->D, ctx->emax,
-                    ctx->arr, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
+    void *ptrs[] = {ctx->state, ctx->row, ctx->fy, ctx->area_d, ctx->ctrl, ctx->dtmin, ctx->scratch_u,
                     ctx->ftime, ctx->gpos, ctx->gbuf, ctx->gtime, ctx->okada_def, ctx->mass_partial, ctx->outbuf,
                     ctx->steplog, ctx->order, ctx->deep, ctx->gen, ctx->fb_count, ctx->fb_list, ctx->health};
     for (void *q : ptrs)

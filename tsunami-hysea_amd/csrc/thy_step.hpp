@@ -93,6 +93,8 @@ template <typename T> struct StepArgs {
     const T *D;
     T *emax;
     T *arr;                  // arrival tracking: eta_ini until the cell arrives, then the arrival time (Arrival<T>)
+    const char *block;       // the arrays above lie in this one allocation (row sweeps: one buffer resource)
+    int block_bytes;         //   its size, below 2^31
     const RowGeo<T> *row;
     const FaceGeoY<T> *fy;
     Ctrl *ctrl;

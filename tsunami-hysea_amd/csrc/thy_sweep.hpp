@@ -129,12 +129,28 @@ template <typename T> struct RowCell {
     T e, D, qx, qy;
 };
 
-// The row sweeps address each array through a buffer resource over the whole array:
-// the lane's column is the vector offset and the row the scalar offset, both 32-bit
-// (every array is under 2^31 bytes; thy_create keeps larger grids off these kernels).
+// The row sweeps address the state arrays through one buffer resource over the state
+// block that holds them all: the lane's column is the vector offset, the array's offset
+// in the block plus the row's the scalar offset, all 32-bit (the block is under 2^31
+// bytes; thy_create keeps larger grids off these kernels).  One resource instead of one
+// per array leaves the scalar registers to the rest of the loop.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t whole_array(const void *p, int bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
 }
+
+// byte offsets in the state block of the arrays one launch reads and writes; without
+// arrival tracking the arrival load reads emax instead (its stores never apply)
+template <typename T> struct StateOffsets {
+    int e, D, qx, qy, oe, oqx, oqy, emax, arr;
+    template <typename A>
+    __device__ __forceinline__ StateOffsets(const A &a, const T *ie, const T *iD, const T *iqx, const T *iqy,
+                                            const T *we, const T *wqx, const T *wqy)
+        : e(at(a, ie)), D(at(a, iD)), qx(at(a, iqx)), qy(at(a, iqy)), oe(at(a, we)), oqx(at(a, wqx)),
+          oqy(at(a, wqy)), emax(at(a, a.emax)), arr(at(a, a.track_arrival ? a.arr : a.emax)) {}
+    template <typename A> static __device__ __forceinline__ int at(const A &a, const T *p) {
+        return (int)((const char *)p - a.block);
+    }
+};
 
 template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
 template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
@@ -227,29 +243,23 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // buffer loads: the row (clamped onto the grid) in the scalar offset, the lane's
         // column in the vector offset; never predicated, so the compiler can keep the
         // prefetches in flight across iterations
-        const int bytes = P * ny * (int)sizeof(T);
-        const __amdgpu_buffer_rsrc_t re = whole_array(ge, bytes), rD = whole_array(gD, bytes);
-        const __amdgpu_buffer_rsrc_t rqx = whole_array(gqx, bytes), rqy = whole_array(gqy, bytes);
-        const __amdgpu_buffer_rsrc_t roe = whole_array(oe, bytes), roqx = whole_array(oqx, bytes);
-        const __amdgpu_buffer_rsrc_t roqy = whole_array(oqy, bytes);
+        const __amdgpu_buffer_rsrc_t rs = whole_array(a.block, a.block_bytes);
+        const StateOffsets<T> o(a, ge, gD, gqx, gqy, oe, oqx, oqy);
         const int vcx = cx * (int)sizeof(T);
         auto load_row = [&](int r) {
             const int so = clamp_row(r) * P * (int)sizeof(T);
             RowCell<T> c;
-            c.e = buf_load<T>(re, vcx, so);
-            c.D = buf_load<T>(rD, vcx, so);
-            c.qx = buf_load<T>(rqx, vcx, so);
-            c.qy = buf_load<T>(rqy, vcx, so);
+            c.e = buf_load<T>(rs, vcx, o.e + so);
+            c.D = buf_load<T>(rs, vcx, o.D + so);
+            c.qx = buf_load<T>(rs, vcx, o.qx + so);
+            c.qy = buf_load<T>(rs, vcx, o.qy + so);
             return c;
         };
-        // metrics inputs: without arrival tracking the arrival load reads emax instead
-        const __amdgpu_buffer_rsrc_t rmax = whole_array(a.emax, bytes);
-        const __amdgpu_buffer_rsrc_t rarr = whole_array(a.track_arrival ? a.arr : a.emax, bytes);
         const int vgx = (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)) * (int)sizeof(T);
         auto load_metrics = [&](int jn, T &m, T &t) {
             const int so = (jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P * (int)sizeof(T);
-            m = buf_load<T>(rmax, vgx, so);
-            t = buf_load<T>(rarr, vgx, so);
+            m = buf_load<T>(rs, vgx, o.emax + so);
+            t = buf_load<T>(rs, vgx, o.arr + so);
         };
 
         // the segment touches the first-order band of the x borders: no x face of it
@@ -425,11 +435,11 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
                     bad = bad || (on && bad_cell(s4, qx, qy));
                 }
-                buf_store<T>(eo, rmax, voff, soff, max_on);
-                buf_store<T>(Arrival<T>::enc(tnow), rarr, voff, soff, arr_on);
-                buf_store<T>(s4, roe, voff, soff, on);
-                buf_store<T>(qx, roqx, voff, soff, on);
-                buf_store<T>(qy, roqy, voff, soff, on);
+                buf_store<T>(eo, rs, voff, o.emax + soff, max_on);
+                buf_store<T>(Arrival<T>::enc(tnow), rs, voff, o.arr + soff, arr_on);
+                buf_store<T>(s4, rs, voff, o.oe + soff, on);
+                buf_store<T>(qx, rs, voff, o.oqx + soff, on);
+                buf_store<T>(qy, rs, voff, o.oqy + soff, on);
             }
             // ---- advance the pipeline ----
             i5 = i4; i4 = i3; i3 = i2; i2 = i1; i1 = i0;
@@ -605,16 +615,9 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         // the strip classification keeps columns x0-4 .. x0+59 and rows y0-4 .. y1+3
         // inside the grid: no clamping.  Loads are buffer loads: the lane's column is the
         // vector offset, the row the scalar offset.
-        const int bytes = P * a.ny * (int)sizeof(T);
-        const __amdgpu_buffer_rsrc_t re = whole_array(c1 ? a.e[1] : a.e[0], bytes);
-        const __amdgpu_buffer_rsrc_t rqx = whole_array(c1 ? a.qx[1] : a.qx[0], bytes);
-        const __amdgpu_buffer_rsrc_t rqy = whole_array(c1 ? a.qy[1] : a.qy[0], bytes);
-        const __amdgpu_buffer_rsrc_t rD = whole_array(a.D, bytes);
-        const __amdgpu_buffer_rsrc_t rmax = whole_array(a.emax, bytes);
-        const __amdgpu_buffer_rsrc_t rarr = whole_array(a.track_arrival ? a.arr : a.emax, bytes);
-        const __amdgpu_buffer_rsrc_t roe = whole_array(oe, bytes);
-        const __amdgpu_buffer_rsrc_t roqx = whole_array(oqx, bytes);
-        const __amdgpu_buffer_rsrc_t roqy = whole_array(oqy, bytes);
+        const __amdgpu_buffer_rsrc_t rs = whole_array(a.block, a.block_bytes);
+        const StateOffsets<T> o(a, c1 ? a.e[1] : a.e[0], a.D, c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], oe, oqx,
+                                oqy);
         const int gx = x0 - SW_HALO + lane;
         const int voff = gx * (int)sizeof(T);
         // byte addresses of the left and right neighbour lanes for ds_bpermute (the same
@@ -622,29 +625,29 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         const int adl = (lane > 0 ? lane - 1 : lane) << 2, adr = (lane < 63 ? lane + 1 : lane) << 2;
         auto left = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adl, __float_as_int(v))); };
         auto right = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(v))); };
-        auto ld = [&](const __amdgpu_buffer_rsrc_t r, const int row) {
-            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, row * P * (int)sizeof(T), 0));
+        auto ld = [&](const int off, const int row) {
+            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, off + row * P * (int)sizeof(T), 0));
         };
         auto load_row = [&](const int r) {
             RowCell<T> c;
-            c.e = ld(re, r);
-            c.D = ld(rD, r);
-            c.qx = ld(rqx, r);
-            c.qy = ld(rqy, r);
+            c.e = ld(o.e, r);
+            c.D = ld(o.D, r);
+            c.qx = ld(o.qx, r);
+            c.qy = ld(o.qy, r);
             return c;
         };
         auto metrics_row = [&](const int j) { return j < y0 ? y0 : (j >= y1 ? y1 - 1 : j); };
         const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT;
         const T hpos = a.hpos, cd2s = T(0.5) * (T(1) - a.cvis);
         // rows y0-4 and y1+3 enter only the branch conditions of the outermost mass faces
-        valid = __all(ld(re, y0 - 4) + ld(rD, y0 - 4) > hpos) && __all(ld(re, y1 + 3) + ld(rD, y1 + 3) > hpos);
+        valid = __all(ld(o.e, y0 - 4) + ld(o.D, y0 - 4) > hpos) && __all(ld(o.e, y1 + 3) + ld(o.D, y1 + 3) > hpos);
         // pipeline registers: input rows r-2, r-1; h* row r-2 with sqrt(h*), qy/h*;
         // mass accumulator of row r-1; momentum accumulators of row r-2
         RowCell<T> i2 = RowCell<T>{T(0), T(0), T(0), T(0)}, i1 = i2;
         T s2 = T(0), sq2 = T(0), v2 = T(0), m1 = T(0);
         T X2 = T(0), Y2 = T(0), W2 = T(0);
         RowCell<T> buf = load_row(y0 - 3);
-        T pm = ld(rmax, y0), pt = ld(rarr, y0);
+        T pm = ld(o.emax, y0), pt = ld(o.arr, y0);
         // One row of the pipeline.  In the steady state (rows y0+2 .. y1) every stage
         // applies, so the stage conditions drop out at compile time; the other rows keep
         // them.  Both forms evaluate the same expressions.
@@ -656,8 +659,8 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             const T cmax = pm, carr = pt;
             {
                 const int jn = kS ? ju + 1 : metrics_row(ju + 1);
-                pm = ld(rmax, jn);
-                pt = ld(rarr, jn);
+                pm = ld(o.emax, jn);
+                pt = ld(o.arr, jn);
             }
             const T h0 = i0.e + i0.D;
             valid = valid && __all(h0 > hpos);
@@ -726,17 +729,17 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 const RowGeo<T> rg = uniform_load(a.row + jr);
                 const int soff = jr * P * (int)sizeof(T);
                 const T D = i2.D, eo = i2.e;
-                buf_store<T>(eo, rmax, voff, soff, on && eo > cmax && eo + D > a.eps_h);
-                buf_store<T>(Arrival<T>::enc(tnow), rarr, voff, soff,
+                buf_store<T>(eo, rs, voff, o.emax + soff, on && eo > cmax && eo + D > a.eps_h);
+                buf_store<T>(Arrival<T>::enc(tnow), rs, voff, o.arr + soff,
                               on && a.track_arrival && !Arrival<T>::arrived(carr) && tabs(eo - carr) > a.dif_at);
                 const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
                 dtmin = on ? fminf(dtmin, (float)dtc) : dtmin;
                 const T val = dt * rg.invA;
                 const T qx = i2.qx + val * X2, qy = i2.qy + val * Y2;
                 bad = bad || (on && bad_cell(s2, qx, qy));
-                buf_store<T>(s2, roe, voff, soff, on);
-                buf_store<T>(qx, roqx, voff, soff, on);
-                buf_store<T>(qy, roqy, voff, soff, on);
+                buf_store<T>(s2, rs, voff, o.oe + soff, on);
+                buf_store<T>(qx, rs, voff, o.oqx + soff, on);
+                buf_store<T>(qy, rs, voff, o.oqy + soff, on);
             }
             i2 = i1; i1 = i0;
             s2 = s1; sq2 = sq1; v2 = v1;
