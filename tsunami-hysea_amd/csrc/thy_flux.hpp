@@ -22,6 +22,8 @@ template <> struct Num<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // a / sqrt(x) with one transcendental (reciprocal square root)
+    static __device__ __forceinline__ float div_sqrt(float a, float x) { return a * __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float exp(float x) {
         return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
     }
@@ -37,6 +39,7 @@ template <> struct Num<double> {
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double div_sqrt(double a, double x) { return a / ::sqrt(x); }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double pow43(double x) { return ::pow(x, 4.0 / 3.0); }
 };
@@ -62,7 +65,7 @@ template <typename T> __device__ __forceinline__ T sgn(T x) {
 template <typename T> __device__ __forceinline__ T desing(T h, T eps_h) {
     T m = tmax(h, eps_h);
     T h2 = h * h, m2 = m * m;
-    return Num<T>::div(T(1.4142135623730951) * h, Num<T>::sqrt(h2 * h2 + m2 * m2));
+    return Num<T>::div_sqrt(T(1.4142135623730951) * h, h2 * h2 + m2 * m2);
 }
 
 // smooth monotone limiter: src/GPU/Arista_kernel.cu:45-68
@@ -75,7 +78,9 @@ template <typename T> __device__ __forceinline__ T limiter(T p0m, T p0, T p1, T 
     if (c0 < Num<T>::eps) return T(1);
     T aux = tabs(Num<T>::div(c1 - c0, c1 + c0));
     c1 = T(0.5) * (c1 + c0) * (T(1) - aux * aux * aux);
-    T vs = (c0 > c1) ? Num<T>::div(c1, c0 + Num<T>::eps) : Num<T>::div(c0, c1 + Num<T>::eps);
+    // the smaller over the larger: one division (select first, then divide)
+    const bool big0 = c0 > c1;
+    T vs = Num<T>::div(big0 ? c1 : c0, (big0 ? c0 : c1) + Num<T>::eps);
     return Num<T>::div(sg * vs * (T(1) + vs), T(1) + vs * vs);
 }
 

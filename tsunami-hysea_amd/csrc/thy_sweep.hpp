@@ -272,6 +272,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // qy/h* of rows r-4..r-2 for the deep y faces), mass accumulators rows
         // r-2..r-1, momentum accumulators rows r-4..r-3, and per-row wavefront votes
         // "every lane deeper than hpos" for the inputs (rows r-3..r) and h* (r-5..r-2)
+        // and "every lane dry" (h <= eps) for the inputs (rows r-2..r) and h* (r-4..r-2)
         RowCell<T> i5, i4, i3, i2, i1, i0;
         i5 = i4 = i3 = i2 = i1 = i0 = RowCell<T>{T(0), T(0), T(0), T(0)};
         T s5 = T(0), s4 = T(0), s3 = T(0);
@@ -280,6 +281,12 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         T X4 = T(0), Y4 = T(0), W4 = T(0), X3 = T(0), Y3 = T(0), W3 = T(0);
         bool din3 = false, din2 = false, din1 = false;
         bool ds5 = false, ds4 = false, ds3 = false;
+        bool dry2 = false, dry1 = false, dsd4 = false, dsd3 = false;
+        // A face both of whose cells are dry (h <= eps) gets a zero flux from the face
+        // functions (src/GPU/Arista_kernel.cu:140,298); where every face of a stage is such
+        // a face (land), the stage uses those zeros without evaluating the faces.
+        constexpr T kZero = T(0);
+        const MF<T> kDryMF{kZero, kZero, kZero, kZero, kZero};
         // One pipeline iteration for row r.  buf holds row r on entry and receives row
         // r+1; (pm, pt) hold the metrics inputs of row r-4 on entry and receive those of
         // row r-3: every load has one iteration of work behind it.  (Keeping loads two
@@ -293,6 +300,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             load_metrics(ju + 1, pm, pt);
             const T h0 = i0.e + i0.D;
             const bool din0 = __all(h0 > hpos) != 0;
+            const bool dry0 = __all(!(h0 > Num<T>::eps)) != 0;
 
             // Stages run only for the rows and faces that some output row of the strip
             // depends on (h* rows y0-2 .. y1+1, momentum faces of rows y0 .. y1-1 and
@@ -304,6 +312,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 if (din0 && !seg_ac) {
                     const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
                     m0 = F - from_right(F);
+                } else if (dry0) {
+                    m0 = ((xb == 1) ? -kZero : kZero) - kZero;
                 } else {
                     const T eL = from_left(i0.e), DL = from_left(i0.D), qL = from_left(i0.qx);
                     const T eLL = from_left2(i0.e), DLL = from_left2(i0.D);
@@ -323,6 +333,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 T F;
                 if (din3 && din2 && din1 && din0 && !yac) {
                     F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
+                } else if (dry2 && dry1) {
+                    F = kZero;
                 } else {
                     const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0));
                     const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0)), Q = sc_of(i0.e, i0.D, T(0), T(0));
@@ -346,6 +358,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             }
             const T hs2 = s2 + i2.D;
             const bool ds2 = hrow && __all(hs2 > hpos) != 0;
+            const bool dsd2 = __all(!(hs2 > Num<T>::eps)) != 0;
             T sq2 = T(0), u2 = T(0), v2 = T(0);
             if (ds2) {
                 sq2 = Num<T>::sqrt(hs2);
@@ -366,6 +379,11 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 X2 = -f.pn - from_right(f.mn);
                 Y2 = -f.pt - from_right(f.mt);
                 W2 = f.w + from_right(f.w);
+            } else if (dsd2) {
+                const MF<T> &f = kDryMF;
+                X2 = -((xb == 1) ? f.mn : f.pn) - f.mn;
+                Y2 = -((xb == 1) ? f.mt : f.pt) - f.mt;
+                W2 = f.w + f.w;
             } else {
                 const T eL = from_left(s2), DL = from_left(i2.D), qxL = from_left(i2.qx), qyL = from_left(i2.qy);
                 const T eLL = from_left2(s2), DLL = from_left2(i2.D), qyLL = from_left2(i2.qy);
@@ -390,6 +408,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
                     B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
                     f = fast_mom_y(A, B, fg, dt, cd2s);
+                } else if (dsd4 && dsd3 && g - 1 >= 0 && g < ny) {
+                    f = kDryMF;  // the face's cells are rows r-4 and r-3 (no clamping here)
                 } else {
                     SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx), A = sc_of(s4, i4.D, i4.qy, i4.qx);
                     SC<T> B = sc_of(s3, i3.D, i3.qy, i3.qx), Q = sc_of(s2, i2.D, i2.qy, i2.qx);
@@ -450,6 +470,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             X3 = X2; Y3 = Y2; W3 = W2;
             din3 = din2; din2 = din1; din1 = din0;
             ds5 = ds4; ds4 = ds3; ds3 = ds2;
+            dry2 = dry1; dry1 = dry0;
+            dsd4 = dsd3; dsd3 = dsd2;
         };
 
         // rows y0-4 .. y1+3
