@@ -1264,7 +1264,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
         // hand-back list starts empty), but publish no control block: every slot holds
         // the same control block, each step reads the same state and redoes the same
         // step, and the run's state stays.
-        constexpr int kSeq = 12;
+        constexpr int kSeq = 32;  // long enough that the replay launch is a small share
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         ctx->tuning = 1;
