@@ -333,7 +333,7 @@ def main():
         v64 = ncell * args.steps / wall64 / 1e9
         fp64 = {"value": v64, "unit": "Gcell-steps/s", "ms_per_step": wall64 * 1e3 / args.steps,
                 "bytes_per_cell_step": 2 * B_ALG[track], "roofline_frac": 2 * B_ALG[track] * v64 / HBM_PEAK_GBS,
-                "kernel": "k_sweep<double> (general faces)"}
+                "kernel": "k_sweep_deep<double> + k_sweep<double> (general, hand-back)"}
         log(f"fp64: {v64:.1f} Gcell-steps/s")
         s64.close()
 

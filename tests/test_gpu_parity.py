@@ -410,24 +410,26 @@ def test_precision_fp32_against_fp64_run():
         s.close()
 
 
-def _split_pair(pr, steps):
+def _split_pair(pr, steps, precision=0):
     out = []
     for kernel in ("sweep", "sweep_general"):
-        s = _native().Solver(pr, kernel=kernel, strip_rows=16)
+        s = _native().Solver(pr, kernel=kernel, strip_rows=16, precision=precision)
         s.run(max_steps=steps, steps_per_sync=6)
         out.append(_fields(s))
         s.close()
     return out
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("borders", [(1, 1, 1, 1), (-1, -1, -1, -1), (1, -1, 2, 2)],
                          ids=["open", "walls", "periodic_x"])
-def test_deep_split_identical_to_general_faces(borders):
+def test_deep_split_identical_to_general_faces(borders, precision):
     """The deep-water launch (interior blocks far from coast and borders) and the
     general launch evaluate the deep faces with the same functions: the split run is
-    bit-identical to the general-faces-only run, with open, wall and periodic borders."""
+    bit-identical to the general-faces-only run, with open, wall and periodic borders,
+    in FP32 and in FP64."""
     pr = make_problem(nx=400, ny=300, seamounts=6, seed=21, borders=borders)
-    a, b = _split_pair(pr, 30)
+    a, b = _split_pair(pr, 30, precision)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
 
@@ -447,11 +449,12 @@ def test_deep_split_hands_back_failing_blocks():
     """A dry patch carved into deep water (classified deep from the still-water depth,
     but h <= hpos at run time): the deep-water launch hands those blocks back to the
     general launch of the same step; the result is still bit-identical to the
-    general-faces-only run and within tolerance of the oracle."""
+    general-faces-only run (FP32 and FP64) and within tolerance of the oracle."""
     pr = _hole_problem()
-    a, b = _split_pair(pr, 20)
-    for x, y in zip(a, b):
-        assert np.array_equal(x, y)
+    for precision in (0, 1):
+        a, b = _split_pair(pr, 20, precision)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), precision
     s = _native().Solver(pr)
     o = _oracle(pr)
     dt = o.initial_dt()

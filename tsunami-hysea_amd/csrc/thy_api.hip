@@ -261,7 +261,7 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
     const int n_chunks = (a.tiles_y + c->chunk_rows - 1) / c->chunk_rows;
     dim3 grid(((n_chunks + 7) / 8) * 8 * a.chunk), block(256);
     if (c->kernel == 0 || c->kernel == 2) {
-        if constexpr (sizeof(T) == 4) {
+        {
             if (a.deep) {
                 // deep-water blocks on the main stream and the general blocks on the side
                 // stream run side by side; after the join, the hand-backs of this step
@@ -274,7 +274,7 @@ hipError_t launch_step_t(thy_ctx *c, int slot) {
                                        c->side, a, slot, 1);
                     if ((e = hipGetLastError()) != hipSuccess) return e;
                 }
-                hipLaunchKernelGGL(k_sweep_deep, dim3(g_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot);
+                hipLaunchKernelGGL((k_sweep_deep<T>), dim3(g_deep), dim3(64 * SW_WAVES), 0, c->stream, a, slot);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
                 if ((e = hipEventRecord(c->ev_join, c->side)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
@@ -452,7 +452,7 @@ int build_order(thy_ctx *ctx, const T *D) {
     // (no clamping, no first-order band), of the sponges, and of coast
     // the deep-water path addresses each array through a buffer resource with a 32-bit
     // byte range
-    const bool split = c->kernel == 0 && c->prec == 0;  // (kernel 0: the state block fits 32-bit offsets)
+    const bool split = c->kernel == 0;  // (kernel 0: the state block fits 32-bit offsets)
     const int spi = c->prm.sponge_izq, spd = c->prm.sponge_der, sps = c->prm.sponge_sup, spn = c->prm.sponge_inf;
     // strips of the split launches: rows shifted up by yoff (the block order of the
     // launch without a split keeps unshifted tiles)

@@ -621,7 +621,6 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
 template <typename T>
 __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<T> &bc, const int x0, const int y0,
                                            float &dtmin_out) {
-    static_assert(sizeof(T) == 4, "the deep-water path is FP32");
     const int lane = threadIdx.x & 63;
     const bool c1 = bc.cur != 0;
     T *__restrict__ oe = c1 ? a.e[0] : a.e[1];
@@ -643,13 +642,18 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         const int gx = x0 - SW_HALO + lane;
         const int voff = gx * (int)sizeof(T);
         // byte addresses of the left and right neighbour lanes for ds_bpermute (the same
-        // moves as from_left / from_right, computed once per strip)
+        // moves as from_left / from_right, computed once per strip; FP64 values move as
+        // two dwords through from_left / from_right)
         const int adl = (lane > 0 ? lane - 1 : lane) << 2, adr = (lane < 63 ? lane + 1 : lane) << 2;
-        auto left = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adl, __float_as_int(v))); };
-        auto right = [&](const T v) { return __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(v))); };
-        auto ld = [&](const int off, const int row) {
-            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, off + row * P * (int)sizeof(T), 0));
+        auto left = [&](const T v) -> T {
+            if constexpr (sizeof(T) == 4) return __int_as_float(__builtin_amdgcn_ds_bpermute(adl, __float_as_int(v)));
+            else return from_left(v);
         };
+        auto right = [&](const T v) -> T {
+            if constexpr (sizeof(T) == 4) return __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(v)));
+            else return from_right(v);
+        };
+        auto ld = [&](const int off, const int row) { return buf_load<T>(rs, voff, off + row * P * (int)sizeof(T)); };
         auto load_row = [&](const int r) {
             RowCell<T> c;
             c.e = ld(o.e, r);
@@ -789,8 +793,9 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
     return valid;
 }
 
-__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<float> a, const int slot) {
-    __shared__ BlockCtl<float> bc;
+template <typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_sweep_deep(const StepArgs<T> a, const int slot) {
+    __shared__ BlockCtl<T> bc;
     __shared__ float s_red[SW_WAVES];
     const int tid = threadIdx.x, bid = blockIdx.x;
     if (tid == 0) {
