@@ -325,6 +325,10 @@ def main():
     if not args.no_fp64:
         s64 = native.Solver(prob, precision=1, track_arrival=track)
         w64 = s64.run(max_steps=args.warmup, steps_per_sync=sync)
+        tuned64 = None
+        if args.kernel == "sweep" and args.strip_rows == 0:
+            rows64, tms64 = s64.autotune(reps=5)  # FP64 has its own best strip heights
+            tuned64 = {"strip_rows": rows64, "general_rows": s64.general_rows, "ms_per_step": tms64}
         s64.run_more(0)
         t0 = time.perf_counter()
         r64 = s64.run_more(args.steps)
@@ -333,7 +337,7 @@ def main():
         v64 = ncell * args.steps / wall64 / 1e9
         fp64 = {"value": v64, "unit": "Gcell-steps/s", "ms_per_step": wall64 * 1e3 / args.steps,
                 "bytes_per_cell_step": 2 * B_ALG[track], "roofline_frac": 2 * B_ALG[track] * v64 / HBM_PEAK_GBS,
-                "kernel": "k_sweep_deep<double> + k_sweep<double> (general, hand-back)"}
+                "kernel": "k_sweep_deep<double> + k_sweep<double> (general, hand-back)", "autotune": tuned64}
         log(f"fp64: {v64:.1f} Gcell-steps/s")
         s64.close()
 
