@@ -334,7 +334,7 @@ void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeo
         r.lenx = (T)al;
         r.dxs = (T)c->ancho[j];
         r.dys = (T)al;
-        r.pad = T(0);
+        r.kA = T(0.25) * r.A;
         rows[j] = r;
     }
     for (int g = 0; g <= ny; g++) {
@@ -349,6 +349,13 @@ void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeo
         f.valy = (T)((2.0 * len) / (c->area[r0] + c->area[r1]));
         f.ic0 = (T)(1.0 / c->cosphi[r0]);
         f.ic1 = (T)(1.0 / c->cosphi[r1]);
+        // in T arithmetic, each a single rounded operation as on the device
+        f.gh = T(0.5) * (T)(f.cos0 + f.cos1);
+        f.gd = (T)(f.cos1 - f.cos0);
+        f.g1ic0 = (T)(f.cos1 * f.ic0);
+        f.k0 = (T)(T(0.25) * f.cos0) * f.A0;
+        f.k1 = (T)(T(0.25) * f.cos1) * f.A1;
+        f.pad0 = f.pad1 = f.pad2 = T(0);
         fys[g] = f;
     }
 }

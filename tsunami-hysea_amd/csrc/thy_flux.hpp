@@ -99,13 +99,14 @@ template <typename T> __device__ __forceinline__ T blend(T hij, T hpos, bool alf
     return T(1);
 }
 
-// positivity limiter: src/GPU/Arista_kernel.cu:79-114 (g0/g1 = 1 for x faces, cos for y).
+// positivity limiter: src/GPU/Arista_kernel.cu:79-114 (g = 1 for x faces, cos for y).
 // The reference limits the flux when dt exceeds dt0 = 0.25 h g A / (|F| + eps); the
-// test is evaluated as dt (|F| + eps) > 0.25 h g A (the same condition, no division
-// unless the limiter acts).  Every face evaluation (deep-water forms and general faces)
-// goes through this one function, so a face evaluated by two wavefronts has one value.
-template <typename T> __device__ __forceinline__ T positivity(T F, T h0, T h1, T g0, T g1, T dt, T A0, T A1) {
-    const T lim = (F > T(0)) ? T(0.25) * h0 * g0 * A0 : T(0.25) * h1 * g1 * A1;
+// test is evaluated as dt (|F| + eps) > h k with k = g A / 4 of the upwind row, taken
+// from the row tables (the same condition, no division unless the limiter acts).  Every
+// face evaluation (deep-water forms and general faces) goes through this one function,
+// so a face evaluated by two wavefronts has one value.
+template <typename T> __device__ __forceinline__ T positivity(T F, T h0, T h1, T k0, T k1, T dt) {
+    const T lim = (F > T(0)) ? h0 * k0 : h1 * k1;
     const T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
     if (dt * den > lim) {
         const T dt0 = Num<T>::div(lim, den);
@@ -140,18 +141,19 @@ template <typename T> __device__ __forceinline__ T orient(int border, const Prm<
 // single adjacent interior cell receives with the sign convention documented in
 // the step kernel.
 //   x: src/GPU/Arista_kernel.cu:118-219,466-563   y: :674-778,1028-1125
-// g0/g1: 1/cos(phi_row) for x faces (both sides), cos0/cos1 for y faces.
+// g0/g1: 1/cos(phi_row) for x faces (both sides), cos0/cos1 for y faces; k0/k1: the
+// positivity factors g A / 4 of the two rows (A / 4 for x faces).
 // ---------------------------------------------------------------------------
 template <typename T, bool YFACE>
 __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int border, bool alfa_cero,
-                                       T len, T g0, T g1, T A0, T A1, T valf, T dt, const Prm<T> &p) {
+                                       T len, T g0, T g1, T k0, T k1, T valf, T dt, const Prm<T> &p) {
     // deep branch (2s): all four depths above hpos and outside the first-order band
     if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
         T F = YFACE ? len * a.qn * g0 : a.qn * len * g0;
-        return YFACE ? positivity(F, a.h, b.h, g0, g1, dt, A0, A1) : positivity(F, a.h, b.h, T(1), T(1), dt, A0, A1);
+        return positivity(F, a.h, b.h, k0, k1, dt);
     }
     T normal = orient(border, p, m, a, b, q);
-    if (border == 1) { T t = g0; g0 = g1; g1 = t; t = A0; A0 = A1; A1 = t; }
+    if (border == 1) { T t = g0; g0 = g1; g1 = t; t = k0; k0 = k1; k1 = t; }
     if (!(a.h > Num<T>::eps || b.h > Num<T>::eps)) return T(0);
     T q0 = a.qn * normal, q1 = b.qn * normal;
     T hij = T(0.5) * (a.h + b.h);
@@ -193,7 +195,7 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
     }
     T alfa = blend(hij, p.hpos, alfa_cero);
     T F = alfa * F2s + (T(1) - alfa) * Fw;
-    return YFACE ? positivity(F, a.h, b.h, g0, g1, dt, A0, A1) : positivity(F, a.h, b.h, T(1), T(1), dt, A0, A1);
+    return positivity(F, a.h, b.h, k0, k1, dt);
 }
 
 // Momentum face result: low cell acc2 -= (mn, mt), high cell acc2 -= (pn, pt),

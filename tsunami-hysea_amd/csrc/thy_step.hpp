@@ -62,11 +62,16 @@ template <typename T> struct Arrival {
 
 template <typename T> struct RowGeo {
     T A, invA, gx, valx;  // area, 1/area, 1/cos(phi), (2*alto/cos)/(2*area)
-    T lenx, dxs, dys, pad; // alto (x-face length), ancho and alto for sponge distances
+    T lenx, dxs, dys, kA;  // alto (x-face length), ancho and alto for sponge distances, A/4
 };
 
 template <typename T> struct FaceGeoY {
     T len, cos0, cos1, A0, A1, valy, ic0, ic1; // ancho, cos/area of rows j-1, j (clamped)
+    // face-uniform factors of the deep-water y momentum flux, computed on the host in the
+    // state type exactly as the general faces compute them: 0.5 (cos0 + cos1),
+    // cos1 - cos0, cos1 / cos0 (as cos1 * ic0); and the positivity limiter's factors
+    // cos A / 4 of the two rows
+    T gh, gd, g1ic0, k0, k1, pad0, pad1, pad2;
 };
 
 template <typename T> struct StepArgs {
@@ -344,10 +349,10 @@ __device__ __forceinline__ void metrics_cell(const StepArgs<T> &a, const int64_t
 
 // x face: F = q0*alto/cos, positivity (src/GPU/Arista_kernel.cu:79-95,118-130)
 template <typename T>
-__device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T A, T dt) {
+__device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T kA, T dt) {
 #pragma clang fp contract(off)
     T F = q0 * il;
-    T lim = (F > T(0)) ? T(0.25) * h0 * A : T(0.25) * h1 * A;
+    T lim = (F > T(0)) ? h0 * kA : h1 * kA;
     T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
     if (dt * den > lim) {
         T dt0 = lim * Num<T>::rcp(den);
@@ -359,10 +364,10 @@ __device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T A, T dt) {
 
 // y face: F = ancho*q0*cos0, positivity with cos (src/GPU/Arista_kernel.cu:97-114,674-686)
 template <typename T>
-__device__ __forceinline__ T deep_mass_y(T q0, T h0, T h1, T len, T c0, T c1, T A0, T A1, T dt) {
+__device__ __forceinline__ T deep_mass_y(T q0, T h0, T h1, T len, T c0, T k0, T k1, T dt) {
 #pragma clang fp contract(off)
     T F = len * q0 * c0;
-    T lim = (F > T(0)) ? T(0.25) * h0 * c0 * A0 : T(0.25) * h1 * c1 * A1;
+    T lim = (F > T(0)) ? h0 * k0 : h1 * k1;
     T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
     if (dt * den > lim) {
         T dt0 = lim * Num<T>::rcp(den);
@@ -487,10 +492,10 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             const T h = e + D;
             const T hl = s_e[io - 1] + s_D[io - 1], hr = s_e[io + 1] + s_D[io + 1];
             const T hb = s_e[io - IX] + s_D[io - IX], ht = s_e[io + IX] + s_D[io + IX];
-            const T Fl = deep_mass_x(s_qx[io - 1], hl, h, rg.lenx * rg.gx, rg.A, dt);
-            const T Fr = deep_mass_x(qx, h, hr, rg.lenx * rg.gx, rg.A, dt);
-            const T Fb = deep_mass_y(s_qy[io - IX], hb, h, fb.len, fb.cos0, fb.cos1, fb.A0, fb.A1, dt);
-            const T Ft = deep_mass_y(qy, h, ht, ft.len, ft.cos0, ft.cos1, ft.A0, ft.A1, dt);
+            const T Fl = deep_mass_x(s_qx[io - 1], hl, h, rg.lenx * rg.gx, rg.kA, dt);
+            const T Fr = deep_mass_x(qx, h, hr, rg.lenx * rg.gx, rg.kA, dt);
+            const T Fb = deep_mass_y(s_qy[io - IX], hb, h, fb.len, fb.cos0, fb.k0, fb.k1, dt);
+            const T Ft = deep_mass_y(qy, h, ht, ft.len, ft.cos0, ft.k0, ft.k1, dt);
             const T acc = (Fl - Fr) + (Fb * fb.ic1 - Ft * ft.ic0);
             const T val = dt * rg.invA;
             T es = e + val * acc;
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const RowGeo<T> rg = a.row[gy];
                 const int lj = sj + 2;
                 SC<T> m = in_cell(fi, lj), c0 = in_cell(fi + 1, lj), c1 = in_cell(fi + 2, lj), q = in_cell(fi + 3, lj);
-                const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.A, rg.A, rg.valx, dt, px);
+                const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.kA, rg.kA, rg.valx, dt, px);
                 const bool lo = fi >= 1 && (per || g - 1 >= 0);
                 const bool hi = fi <= SX - 1 && (per || g <= nx - 1);
                 if (lo) s_am[sj * SX + fi - 1] -= F;
@@ -672,7 +677,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 SC<T> m = in_cell(li, fj), c0 = in_cell(li, fj + 1), c1 = in_cell(li, fj + 2), q = in_cell(li, fj + 3);
                 m.qn = s_qy[fj * IX + li]; c0.qn = s_qy[(fj + 1) * IX + li];
                 c1.qn = s_qy[(fj + 2) * IX + li]; q.qn = s_qy[(fj + 3) * IX + li];
-                const T F = mass_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.A0, fg.A1, fg.valy, dt, py);
+                const T F = mass_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.k0, fg.k1, fg.valy, dt, py);
                 const bool lo = fj >= 1 && g - 1 >= 0;
                 const bool hi = fj <= SY - 1 && g <= ny - 1;
                 if (lo) s_am[(fj - 1) * SX + si] -= F * fg.ic0;

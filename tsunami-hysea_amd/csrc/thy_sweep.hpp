@@ -53,16 +53,16 @@ template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
 // ---------------------------------------------------------------------------
 // x face between cells a (left) and b: src/GPU/Arista_kernel.cu:79-95,118-130
 template <typename T>
-__device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T len, T g, T A, T dt) {
+__device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T len, T g, T kA, T dt) {
     const T F = qa * len * g;
-    return positivity(F, ha, hb, T(1), T(1), dt, A, A);
+    return positivity(F, ha, hb, kA, kA, dt);
 }
 
 // y face between rows a (g-1) and b (g): src/GPU/Arista_kernel.cu:97-114,674-686
 template <typename T>
 __device__ __forceinline__ T fast_mass_y(T qa, T ha, T hb, const FaceGeoY<T> &f, T dt) {
     const T F = f.len * qa * f.cos0;
-    return positivity(F, ha, hb, f.cos0, f.cos1, dt, f.A0, f.A1);
+    return positivity(F, ha, hb, f.k0, f.k1, dt);
 }
 
 // one cell of a deep momentum face: total depth, eta, depth, normal and tangential
@@ -104,7 +104,7 @@ __device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, cons
     const T hij = T(0.5) * (a.h + b.h);
     const T uij = Num<T>::div(a.s * a.un + b.s * b.un, a.s + b.s + Num<T>::eps);
     const T lmax = tabs(uij) + Num<T>::sqrt(hij);
-    const T deta = T(0.5) * (g0 + g1) * deta_of(a.e, b.e, a.D, b.D) - uij * uij * (g1 - g0);
+    const T deta = f.gh * deta_of(a.e, b.e, a.D, b.D) - uij * uij * f.gd;
     const T val = f.valy * dt;
     const bool up = uij < T(0);
     const T uu = up ? b.un : a.un;
@@ -112,7 +112,7 @@ __device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, cons
     const T qtt = up ? b.qt : a.qt;
     const T fn = uu * qnn * len;
     T mn, pn;
-    if (up) { pn = -fn; mn = fn * (g1 * ic0); }
+    if (up) { pn = -fn; mn = fn * f.g1ic0; }
     else { mn = fn; pn = -fn * g0 * ic1; }
     mn += hij * deta * len * ic0;
     const T DES = cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
@@ -310,7 +310,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             if (r >= y0 - 2 && r <= y1 + 1) {
                 const RowGeo<T> rg = uniform_load(a.row + clamp_row(r));
                 if (din0 && !seg_ac) {
-                    const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
+                    const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
                     m0 = F - from_right(F);
                 } else if (dry0) {
                     m0 = ((xb == 1) ? -kZero : kZero) - kZero;
@@ -320,7 +320,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     const T eR = from_right(i0.e), DR = from_right(i0.D);
                     const SC<T> M = sc_of(eLL, DLL, T(0), T(0)), A = sc_of(eL, DL, qL, T(0));
                     const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0)), Q = sc_of(eR, DR, T(0), T(0));
-                    const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.lenx, rg.gx, rg.gx, rg.A, rg.A, rg.valx, dt, px);
+                    const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.lenx, rg.gx, rg.gx, rg.kA, rg.kA, rg.valx, dt, px);
                     m0 = ((xb == 1) ? -F : F) - from_right(F);
                 }
             }
@@ -338,7 +338,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 } else {
                     const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0));
                     const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0)), Q = sc_of(i0.e, i0.D, T(0), T(0));
-                    F = mass_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.A0, fg.A1, fg.valy, dt, py);
+                    F = mass_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.k0, fg.k1, fg.valy, dt, py);
                 }
                 m2 -= F * fg.ic0;
                 m1 += ((yb == 1) ? -F : F) * fg.ic1;
@@ -694,7 +694,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             T m0 = T(0);
             if (kS || (r >= y0 - 2 && r <= y1 + 1)) {
                 const RowGeo<T> rg = uniform_load(a.row + r);
-                const T F = fast_mass_x(left(i0.qx), left(h0), h0, rg.lenx, rg.gx, rg.A, dt);
+                const T F = fast_mass_x(left(i0.qx), left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
                 m0 = F - right(F);
             }
             // mass y-face g = r between rows r-1 and r (faces y0-2 .. y1+2)
