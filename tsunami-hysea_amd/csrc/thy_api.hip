@@ -1716,6 +1716,21 @@ int batch_items(thy_batch *b) {
         for (size_t p = 0; p < gl.size(); p++)
             gv.push_back({gc[p], make_int2(i, (gl[p] << 1) | (nd && p == 0 ? 1 : 0))});
     }
+    // the deep-water items as in a run alone (build_order): row-major, scenario by
+    // scenario, cut into 8 runs of (nearly) equal length, one per XCD, padded to whole
+    // workgroups with idle items (scenario -1), so that each XCD sweeps one band and
+    // strips side by side share their halo columns in its L2
+    if (!b->h_deep.empty()) {
+        const std::vector<int2> rm = b->h_deep;
+        const int nd = (int)rm.size(), q = nd / 8, rem = nd % 8;
+        const int wmax = (q + (rem > 0 ? 1 : 0) + SW_WAVES - 1) / SW_WAVES;
+        b->h_deep.assign((size_t)wmax * 8 * SW_WAVES, make_int2(-1, 0));
+        for (int x = 0; x < 8; x++) {
+            const int size = q + (x < rem ? 1 : 0), start = x * q + std::min(x, rem);
+            for (int i = 0; i < size; i++)
+                b->h_deep[((size_t)(i / SW_WAVES) * 8 + x) * SW_WAVES + i % SW_WAVES] = rm[start + i];
+        }
+    }
     std::stable_sort(gv.begin(), gv.end(), [](const G &x, const G &y) { return x.cls > y.cls; });
     b->h_gen.resize(gv.size());
     for (size_t k = 0; k < gv.size(); k++) b->h_gen[k] = gv[k].it;

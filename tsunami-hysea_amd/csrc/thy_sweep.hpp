@@ -880,6 +880,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_batch_deep(const BatchArgs q,
     if (k >= n) return;
     const int2 w = q.deep[k];
     const int sc = __builtin_amdgcn_readfirstlane(w.x), code = __builtin_amdgcn_readfirstlane(w.y);
+    if (sc < 0) return;  // padding of an XCD run
     const StepArgs<float> a = uniform_load(q.args + sc);
     if (!a.ctrl) return;
     const BlockCtl<float> bc = wave_begin(a, slot, (code & 1) != 0);
