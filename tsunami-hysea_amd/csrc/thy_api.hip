@@ -10,7 +10,8 @@
 //   emax, arr            metrics (arr, when arrival times are tracked: eta_ini until a
 //                        cell arrives, then its arrival time; see Arrival<T>)
 //   row[ny], fy[ny+1]    per-row / per-face-row geometry constants
-//   ctrl[3], dtmin[3]    device control block and dt slots
+//   ctrl[3], dtmin       device control blocks and dt minima (3 launch slots of
+//                        kDtParts partials, dt_at)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,10 +62,11 @@ struct thy_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // two replays in flight: start / end / control-copied events, pinned copies of the
-    // control block, the raw dt slot and the health flag after each replay
+    // control block, the last launch's dt partials and the health flag after each replay
     hipEvent_t rev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     Ctrl *h_snap = nullptr;
-    unsigned int *h_aux = nullptr;     // [2][2]: raw dt slot, health flag
+    unsigned int *h_aux = nullptr;     // [2]: health flag
+    unsigned int *h_dt = nullptr;      // [2][kDtSlot]: dt partials of one launch slot
     int slot = 0;
     int cur = 0;                 // host mirror of ctrl.cur after a sync
     int tx = 64, ty = 16;
@@ -704,9 +706,32 @@ int write_ctrl(thy_ctx *ctx, const Ctrl &c) {
     return THY_OK;
 }
 
+// minimum over the dt partials of one launch slot, as copied to the host (kDtSlot words)
+unsigned int dt_parts_min(const unsigned int *w) {
+    unsigned int m = kDtNone;
+    for (int p = 0; p < kDtParts; p++) m = std::min(m, w[dt_at(0, p)]);
+    return m;
+}
+
+// the raw dt minimum of launch slot s
+int read_dt_slot(thy_ctx *ctx, const int s, unsigned int &raw) {
+    std::vector<unsigned int> w(kDtSlot);
+    HIPCHK(ctx_copy(ctx, w.data(), ctx->dtmin + dt_at(s, 0), kDtSlot * sizeof(unsigned int), hipMemcpyDeviceToHost));
+    raw = dt_parts_min(w.data());
+    return THY_OK;
+}
+
+// every partial of launch slot s (all three slots for s < 0) back to "no cell reported"
+int clear_dt_slots(thy_ctx *ctx, const int s) {
+    const std::vector<unsigned int> v((s < 0 ? 3 : 1) * kDtSlot, kDtNone);
+    HIPCHK(ctx_copy(ctx, ctx->dtmin + (s < 0 ? 0 : dt_at(s, 0)), v.data(), v.size() * sizeof(unsigned int),
+                    hipMemcpyHostToDevice));
+    return THY_OK;
+}
+
 int reset_dt_slots(thy_ctx *ctx) {
-    unsigned int v[3] = {0x7f7fffffu, 0x7f7fffffu, 0x7f7fffffu};
-    HIPCHK(ctx_copy(ctx, ctx->dtmin, v, sizeof(v), hipMemcpyHostToDevice));
+    int r;
+    if ((r = clear_dt_slots(ctx, -1)) != THY_OK) return r;
     if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
     return THY_OK;
 }
@@ -887,8 +912,8 @@ int finalise_on_host(thy_ctx *ctx, Ctrl &c, const unsigned int raw) {
 }
 
 // Advance until the stop predicate holds; graph replays of K steps.  After each replay
-// the control block, the raw dt slot and the health flag are copied to pinned memory
-// behind it.  While the last replay's time, steps and mean dt say that the run goes on
+// the control block, the dt partials of its last launch and the health flag are copied
+// to pinned memory behind it.  While the last replay's time, steps and mean dt say that the run goes on
 // for two more full replays (before t_end, max_steps, the next fault and the next
 // snapshot), the next replay is queued before the one in flight is waited for, so that
 // the device does not idle between replays.  When the host sees the stop predicate hold
@@ -901,8 +926,7 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
     if ((r = read_ctrl(ctx, &last)) != THY_OK) return r;
     if (must_stop(last, ctx->ftimes.data())) {  // stopped before any step
         unsigned int raw = 0;
-        if (!last.dt_valid)
-            HIPCHK(ctx_copy(ctx, &raw, &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost));
+        if (!last.dt_valid && (r = read_dt_slot(ctx, (ctx->slot + 2) % 3, raw)) != THY_OK) return r;
         if ((r = finalise_on_host(ctx, last, raw)) != THY_OK) return r;
         *out = last;
         if (ms) *ms = 0.0;
@@ -921,10 +945,10 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         if ((e = hipMemcpyAsync(&ctx->h_snap[i], &ctx->ctrl[ctx->slot], sizeof(Ctrl), hipMemcpyDeviceToHost,
                                 ctx->stream)) != hipSuccess)
             return e;
-        if ((e = hipMemcpyAsync(&ctx->h_aux[2 * i], &ctx->dtmin[(ctx->slot + 2) % 3], 4, hipMemcpyDeviceToHost,
-                                ctx->stream)) != hipSuccess)
+        if ((e = hipMemcpyAsync(ctx->h_dt + (size_t)i * kDtSlot, ctx->dtmin + dt_at((ctx->slot + 2) % 3, 0),
+                                kDtSlot * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
             return e;
-        if ((e = hipMemcpyAsync(&ctx->h_aux[2 * i + 1], ctx->health, 4, hipMemcpyDeviceToHost, ctx->stream)) !=
+        if ((e = hipMemcpyAsync(&ctx->h_aux[i], ctx->health, 4, hipMemcpyDeviceToHost, ctx->stream)) !=
             hipSuccess)
             return e;
         if ((e = hipEventRecord(ctx->rev[i][2], ctx->stream)) != hipSuccess) return e;
@@ -944,8 +968,8 @@ int advance(thy_ctx *ctx, int K, Ctrl *out, double *ms) {
         hipError_t e = hipEventSynchronize(ctx->rev[head][2]);
         if (e != hipSuccess) return e;
         c = ctx->h_snap[head];
-        raw = ctx->h_aux[2 * head];
-        if (ctx->h_aux[2 * head + 1]) c.nonfinite = 1;
+        raw = dt_parts_min(ctx->h_dt + (size_t)head * kDtSlot);
+        if (ctx->h_aux[head]) c.nonfinite = 1;
         float f = 0.f;
         (void)hipEventElapsedTime(&f, ctx->rev[head][0], ctx->rev[head][1]);
         total += f;
@@ -1110,6 +1134,7 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     }
     HIPCHK(Pool::get().host_alloc((void **)&ctx->h_snap, 2 * sizeof(Ctrl)));
     HIPCHK(Pool::get().host_alloc((void **)&ctx->h_aux, 4 * sizeof(unsigned int)));
+    HIPCHK(Pool::get().host_alloc((void **)&ctx->h_dt, 2 * kDtSlot * sizeof(unsigned int)));
     const size_t n = (size_t)ctx->pitch * ctx->ny, b = n * ctx->esz;
     HIPCHK(dev_alloc(&ctx->state, ctx->state_bytes));
     {
@@ -1126,7 +1151,7 @@ int thy_create(const thy_grid *g, const thy_params *p, thy_ctx **out) {
     HIPCHK(dev_alloc((void **)&ctx->okada_def, (size_t)ctx->nx * ctx->ny * sizeof(double)));
     HIPCHK(dev_alloc((void **)&ctx->outbuf, (size_t)ctx->nx * ctx->ny * sizeof(float)));
     HIPCHK(dev_alloc((void **)&ctx->ctrl, 3 * sizeof(Ctrl)));
-    HIPCHK(dev_alloc((void **)&ctx->dtmin, 3 * sizeof(unsigned int)));
+    HIPCHK(dev_alloc((void **)&ctx->dtmin, 3 * kDtSlot * sizeof(unsigned int)));
     HIPCHK(dev_alloc((void **)&ctx->scratch_u, 4 * sizeof(unsigned int)));
     HIPCHK(dev_alloc((void **)&ctx->health, sizeof(unsigned int)));
     HIPCHK(ctx_fill(ctx, ctx->health, sizeof(unsigned int)));
@@ -1191,6 +1216,7 @@ void thy_destroy(thy_ctx *ctx) {
             if (ctx->rev[i][j]) (void)hipEventDestroy(ctx->rev[i][j]);
     Pool::get().host_free(ctx->h_snap);
     Pool::get().host_free(ctx->h_aux);
+    Pool::get().host_free(ctx->h_dt);
     if (ctx->stream) Pool::get().put_streams(ctx->stream, ctx->side);
     delete ctx;
 }
@@ -1243,10 +1269,10 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     int r;
     // the run's control blocks and dt slots are restored afterwards
     Ctrl saved[3];
-    unsigned int saved_dt[3];
+    std::vector<unsigned int> saved_dt(3 * kDtSlot);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx_copy(ctx, saved, ctx->ctrl, sizeof(saved), hipMemcpyDeviceToHost));
-    HIPCHK(ctx_copy(ctx, saved_dt, ctx->dtmin, sizeof(saved_dt), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_copy(ctx, saved_dt.data(), ctx->dtmin, saved_dt.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
     const int saved_slot = ctx->slot;
     double dt;
     if ((r = thy_initial_dt(ctx, &dt)) != THY_OK) return r;
@@ -1324,7 +1350,7 @@ int thy_autotune(thy_ctx *ctx, int reps, int *strip_rows, int *general_rows, dou
     if (r != THY_OK) return r;
     ctx->slot = saved_slot;
     HIPCHK(ctx_copy(ctx, ctx->ctrl, saved, sizeof(saved), hipMemcpyHostToDevice));
-    HIPCHK(ctx_copy(ctx, ctx->dtmin, saved_dt, sizeof(saved_dt), hipMemcpyHostToDevice));
+    HIPCHK(ctx_copy(ctx, ctx->dtmin, saved_dt.data(), saved_dt.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
     // the hand-back counts as a run leaves them between steps (the next launch resets its own)
     if (ctx->fb_count) HIPCHK(ctx_fill(ctx, ctx->fb_count, 3 * sizeof(unsigned int)));
     if (strip_rows) *strip_rows = best;
@@ -1342,9 +1368,8 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
     c.dt_valid = 1;
     c.cur = ctx->cur;
     if ((r = write_ctrl(ctx, c)) != THY_OK) return r;
-    unsigned int big = 0x7f7fffffu;
     const int s = ctx->slot;
-    HIPCHK(ctx_copy(ctx, &ctx->dtmin[s], &big, sizeof(big), hipMemcpyHostToDevice));
+    if ((r = clear_dt_slots(ctx, s)) != THY_OK) return r;
     if (ctx->fb_count) HIPCHK(ctx_fill(ctx, &ctx->fb_count[s], sizeof(unsigned int)));
     hipError_t e = launch_step(ctx);
     if (e != hipSuccess) return fail(ctx, THY_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
@@ -1357,7 +1382,7 @@ int thy_step(thy_ctx *ctx, double dt, double *dt_next) {
     if (bad) return fail(ctx, THY_ERR_NONFINITE, "the step produced a non-finite eta, qx or qy");
     if (dt_next) {
         unsigned int v;
-        HIPCHK(ctx_copy(ctx, &v, &ctx->dtmin[s], sizeof(v), hipMemcpyDeviceToHost));
+        if ((r = read_dt_slot(ctx, s, v)) != THY_OK) return r;
         float f;
         memcpy(&f, &v, 4);
         *dt_next = (double)f;

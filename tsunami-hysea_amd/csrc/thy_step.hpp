@@ -74,6 +74,24 @@ template <typename T> struct FaceGeoY {
     T gh, gd, g1ic0, k0, k1, pad0, pad1, pad2;
 };
 
+// The dt minimum of a launch is kept as kDtParts partial minima (float bits; for
+// positive floats the unsigned order is the float order), one 64-byte line apart:
+// atomics on one line are applied one after another.  The launches of one grid fold
+// one minimum per workgroup into partial 0, and the next launch reads that one word.
+// The batch launches fold one per wavefront (several thousand per launch for a large
+// grid) into partial (wavefront index mod kDtParts), so that the launch does not wait
+// for a queue of atomics on one line, and read all partials.  The writer of a launch
+// resets every partial of the next launch's slot.
+constexpr int kDtParts = 8;
+constexpr int kDtStride = 16;  // words: one 64-byte line per partial
+constexpr int kDtSlot = kDtParts * kDtStride;
+constexpr unsigned int kDtNone = 0x7f7fffffu;  // FLT_MAX: no cell has reported yet
+
+// word of partial `part` (>= 0) of launch slot `slot` (0..2)
+__host__ __device__ __forceinline__ int dt_at(const int slot, const int part) {
+    return slot * kDtSlot + (part % kDtParts) * kDtStride;
+}
+
 template <typename T> struct StepArgs {
     int nx, ny, pitch, periodic;
     int tiles_x, tiles_y, n_tiles, chunk;  // chunk: tiles per XCD chunk (whole tile rows)
@@ -103,7 +121,7 @@ template <typename T> struct StepArgs {
     const RowGeo<T> *row;
     const FaceGeoY<T> *fy;
     Ctrl *ctrl;
-    unsigned int *dtmin;     // 3 slots, float bits
+    unsigned int *dtmin;     // 3 launch slots of kDtParts partial minima (dt_at)
     const double *ftime;     // fault times (s), sorted
     const int *gpos;
     float *gbuf;             // [gcap][n_gauges][3]
@@ -166,16 +184,20 @@ template <typename T> struct BlockCtl {
 };
 
 // Start of a step launch (thread 0 of every workgroup): read the control block of this
-// launch, finalise dt (truncation at fault times, src/GPU/Timestep.cu:35-68) and the
-// stop predicate; the writer (workgroup 0) also logs the finished iteration, resets
-// the dt slot of the next launch and publishes the next control block.
+// launch, finalise dt from the previous launch's minimum (the least of its first
+// `parts` dt partials) with the truncation at fault times (src/GPU/Timestep.cu:35-68)
+// and the stop predicate; the writer (workgroup 0) also logs the finished iteration,
+// resets the dt partials of the next launch and publishes the next control block.
 template <typename T>
-__device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot, const bool writer, BlockCtl<T> &bc) {
+__device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot, const bool writer, BlockCtl<T> &bc,
+                                           const int parts = 1) {
     Ctrl c = a.ctrl[slot];
     // the health flag raised by the previous step's cells stops the run here
     if (a.health && *a.health) c.nonfinite = 1;
     int skip;
-    const double dt = step_dt(c, a.ftime, __uint_as_float(a.dtmin[(slot + 2) % 3]), skip);
+    unsigned int raw = kDtNone;
+    for (int p = 0; p < parts; p++) raw = min(raw, a.dtmin[dt_at((slot + 2) % 3, p)]);
+    const double dt = step_dt(c, a.ftime, __uint_as_float(raw), skip);
     const bool stop = must_stop(c, a.ftime);
     const bool sample = !stop && (c.t >= c.next_series);
     bc.t = c.t;
@@ -187,7 +209,7 @@ __device__ __forceinline__ void begin_step(const StepArgs<T> &a, const int slot,
     bc.gcap = c.gcap;
     bc.Tscale = c.T;
     if (writer) {
-        a.dtmin[(slot + 1) % 3] = 0x7f7fffffu;  // FLT_MAX: reset the slot of the next launch
+        for (int p = 0; p < kDtParts; p++) a.dtmin[dt_at((slot + 1) % 3, p)] = kDtNone;  // the next launch's partials
         if (a.no_publish) return;
         // iteration c.steps has finished: record the dt chosen for the next one and the time
         if (c.steps > 0 && a.steplog) {
@@ -602,7 +624,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 float m = s_red[0];
 #pragma unroll
                 for (int w = 1; w < NT / 64; w++) m = fminf(m, s_red[w]);
-                atomicMin(&a.dtmin[slot], __float_as_uint(m));
+                atomicMin(&a.dtmin[dt_at(slot, 0)], __float_as_uint(m));
             }
             return;
         }
@@ -846,7 +868,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         float m = s_red[0];
 #pragma unroll
         for (int w = 1; w < NT / 64; w++) m = fminf(m, s_red[w]);
-        atomicMin(&a.dtmin[slot], __float_as_uint(m));
+        atomicMin(&a.dtmin[dt_at(slot, 0)], __float_as_uint(m));
     }
 }
 

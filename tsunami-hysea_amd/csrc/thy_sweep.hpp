@@ -535,7 +535,7 @@ __device__ __forceinline__ void block_dt_min(const StepArgs<T> &a, const int slo
         float m = s_red[0];
 #pragma unroll
         for (int w = 1; w < SW_WAVES; w++) m = fminf(m, s_red[w]);
-        atomicMin(&a.dtmin[slot], __float_as_uint(m));
+        atomicMin(&a.dtmin[dt_at(slot, 0)], __float_as_uint(m));
     }
 }
 
@@ -859,7 +859,7 @@ struct BatchArgs {
 __device__ __forceinline__ BlockCtl<float> wave_begin(const StepArgs<float> &a, const int slot, const bool writer) {
     const int lane = threadIdx.x & 63;
     BlockCtl<float> bc;
-    begin_step(a, slot, writer && lane == 0, bc);
+    begin_step(a, slot, writer && lane == 0, bc, kDtParts);
     if (writer && bc.run) {
         const bool c1 = bc.cur != 0;
         gather_gauges(a, bc, c1 ? a.e[1] : a.e[0], c1 ? a.qx[1] : a.qx[0], c1 ? a.qy[1] : a.qy[0], lane, 64);
@@ -869,7 +869,8 @@ __device__ __forceinline__ BlockCtl<float> wave_begin(const StepArgs<float> &a, 
 
 __device__ __forceinline__ void wave_dt_min(const StepArgs<float> &a, const int slot, const float dtmin) {
     const float m = wave_min(dtmin);
-    if ((threadIdx.x & 63) == 0) atomicMin(&a.dtmin[slot], __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0)
+        atomicMin(&a.dtmin[dt_at(slot, blockIdx.x * SW_WAVES + (threadIdx.x >> 6))], __float_as_uint(m));
 }
 
 __global__ void __launch_bounds__(64 * SW_WAVES) k_batch_deep(const BatchArgs q, const int slot) {
