@@ -36,6 +36,28 @@ template <typename T> __device__ __forceinline__ T from_left(T v) { return __shf
 template <typename T> __device__ __forceinline__ T from_left2(T v) { return __shfl_up(v, 2, 64); }
 template <typename T> __device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
 
+// The same moves as VALU data-parallel-primitive moves across the whole wavefront
+// (wave_shr:1 / wave_shl:1; the lane without a source keeps its own value): a few cycles
+// of latency instead of an LDS round trip.  The general strips, whose wavefronts are few
+// and walk long dependency chains, use these; the deep-water strips keep the LDS moves,
+// which leave the VALU to their flux arithmetic.  (dpp_left2 differs from from_left2 only
+// in lane 1, which lies outside every output cell's stencil.)
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float dpp_left(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+__device__ __forceinline__ float dpp_right(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+__device__ __forceinline__ double dpp_left(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shr1((int)(b & 0xffffffffll)), hi = dpp_shr1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dpp_right(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shl1((int)(b & 0xffffffffll)), hi = dpp_shl1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T> __device__ __forceinline__ T dpp_left2(T v) { return dpp_left(dpp_left(v)); }
+
 // read-only table entry at a wavefront-uniform address (scalar load)
 template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
     static_assert(sizeof(S) % 4 == 0, "table entries are whole dwords");
@@ -310,18 +332,18 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             if (r >= y0 - 2 && r <= y1 + 1) {
                 const RowGeo<T> rg = uniform_load(a.row + clamp_row(r));
                 if (din0 && !seg_ac) {
-                    const T F = fast_mass_x(from_left(i0.qx), from_left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
-                    m0 = F - from_right(F);
+                    const T F = fast_mass_x(dpp_left(i0.qx), dpp_left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
+                    m0 = F - dpp_right(F);
                 } else if (dry0) {
                     m0 = ((xb == 1) ? -kZero : kZero) - kZero;
                 } else {
-                    const T eL = from_left(i0.e), DL = from_left(i0.D), qL = from_left(i0.qx);
-                    const T eLL = from_left2(i0.e), DLL = from_left2(i0.D);
-                    const T eR = from_right(i0.e), DR = from_right(i0.D);
+                    const T eL = dpp_left(i0.e), DL = dpp_left(i0.D), qL = dpp_left(i0.qx);
+                    const T eLL = dpp_left2(i0.e), DLL = dpp_left2(i0.D);
+                    const T eR = dpp_right(i0.e), DR = dpp_right(i0.D);
                     const SC<T> M = sc_of(eLL, DLL, T(0), T(0)), A = sc_of(eL, DL, qL, T(0));
                     const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0)), Q = sc_of(eR, DR, T(0), T(0));
                     const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.lenx, rg.gx, rg.gx, rg.kA, rg.kA, rg.valx, dt, px);
-                    m0 = ((xb == 1) ? -F : F) - from_right(F);
+                    m0 = ((xb == 1) ? -F : F) - dpp_right(F);
                 }
             }
             // ---- mass y-face g = r-1 (rows r-3..r) ----
@@ -372,29 +394,29 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 // halo row: no output row uses its x faces
             } else if (ds2 && !seg_ac) {
                 DC<T> A, B;
-                A.h = from_left(hs2); A.e = from_left(s2); A.D = from_left(i2.D);
-                A.qn = from_left(i2.qx); A.qt = from_left(i2.qy); A.s = from_left(sq2); A.un = from_left(u2);
+                A.h = dpp_left(hs2); A.e = dpp_left(s2); A.D = dpp_left(i2.D);
+                A.qn = dpp_left(i2.qx); A.qt = dpp_left(i2.qy); A.s = dpp_left(sq2); A.un = dpp_left(u2);
                 B.h = hs2; B.e = s2; B.D = i2.D; B.qn = i2.qx; B.qt = i2.qy; B.s = sq2; B.un = u2;
                 const MF<T> f = fast_mom_x(A, B, rg2.lenx, rg2.gx, rg2.valx, dt, cd2s);
-                X2 = -f.pn - from_right(f.mn);
-                Y2 = -f.pt - from_right(f.mt);
-                W2 = f.w + from_right(f.w);
+                X2 = -f.pn - dpp_right(f.mn);
+                Y2 = -f.pt - dpp_right(f.mt);
+                W2 = f.w + dpp_right(f.w);
             } else if (dsd2) {
                 const MF<T> &f = kDryMF;
                 X2 = -((xb == 1) ? f.mn : f.pn) - f.mn;
                 Y2 = -((xb == 1) ? f.mt : f.pt) - f.mt;
                 W2 = f.w + f.w;
             } else {
-                const T eL = from_left(s2), DL = from_left(i2.D), qxL = from_left(i2.qx), qyL = from_left(i2.qy);
-                const T eLL = from_left2(s2), DLL = from_left2(i2.D), qyLL = from_left2(i2.qy);
-                const T eR = from_right(s2), DR = from_right(i2.D), qyR = from_right(i2.qy);
+                const T eL = dpp_left(s2), DL = dpp_left(i2.D), qxL = dpp_left(i2.qx), qyL = dpp_left(i2.qy);
+                const T eLL = dpp_left2(s2), DLL = dpp_left2(i2.D), qyLL = dpp_left2(i2.qy);
+                const T eR = dpp_right(s2), DR = dpp_right(i2.D), qyR = dpp_right(i2.qy);
                 const SC<T> M = sc_of(eLL, DLL, T(0), qyLL), A = sc_of(eL, DL, qxL, qyL);
                 const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy), Q = sc_of(eR, DR, T(0), qyR);
                 const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.lenx, rg2.gx, rg2.gx, T(1), T(1),
                                                    rg2.valx, dt, px);
-                X2 = -((xb == 1) ? f.mn : f.pn) - from_right(f.mn);
-                Y2 = -((xb == 1) ? f.mt : f.pt) - from_right(f.mt);
-                W2 = f.w + from_right(f.w);
+                X2 = -((xb == 1) ? f.mn : f.pn) - dpp_right(f.mn);
+                Y2 = -((xb == 1) ? f.mt : f.pt) - dpp_right(f.mt);
+                W2 = f.w + dpp_right(f.w);
             }
             // ---- momentum y-face g = r-3 (rows r-5..r-2) ----
             if (r - 3 >= y0 && r - 3 <= y1) {
