@@ -301,9 +301,12 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         T sq4 = T(0), sq3 = T(0), v4 = T(0), v3 = T(0);
         T m2 = T(0), m1 = T(0);
         T X4 = T(0), Y4 = T(0), W4 = T(0), X3 = T(0), Y3 = T(0), W3 = T(0);
-        bool din3 = false, din2 = false, din1 = false;
-        bool ds5 = false, ds4 = false, ds3 = false;
-        bool dry2 = false, dry1 = false, dsd4 = false, dsd3 = false;
+        // the wavefront votes of the rows in the pipeline, one bit each in one word (one
+        // scalar register instead of one per vote): bit k of each byte is the vote of the
+        // k-th row back; byte 0 "deeper than hpos" (inputs, rows r..r-3), byte 1 "dry"
+        // (inputs, rows r..r-2), byte 2 "deeper than hpos" (h*, rows r-2..r-5), byte 3 "dry"
+        // (h*, rows r-2..r-4)
+        unsigned int votes = 0u;
         // A face both of whose cells are dry (h <= eps) gets a zero flux from the face
         // functions (src/GPU/Arista_kernel.cu:140,298); where every face of a stage is such
         // a face (land), the stage uses those zeros without evaluating the faces.
@@ -323,6 +326,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             const T h0 = i0.e + i0.D;
             const bool din0 = __all(h0 > hpos) != 0;
             const bool dry0 = __all(!(h0 > Num<T>::eps)) != 0;
+            votes |= (din0 ? 0x1u : 0u) | (dry0 ? 0x100u : 0u);
 
             // Stages run only for the rows and faces that some output row of the strip
             // depends on (h* rows y0-2 .. y1+1, momentum faces of rows y0 .. y1-1 and
@@ -353,9 +357,9 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
                 const bool yac = (g < 2) || (g > ny - 2);
                 T F;
-                if (din3 && din2 && din1 && din0 && !yac) {
+                if ((votes & 0xfu) == 0xfu && !yac) {
                     F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
-                } else if (dry2 && dry1) {
+                } else if ((votes & 0x600u) == 0x600u) {
                     F = kZero;
                 } else {
                     const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0));
@@ -381,6 +385,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             const T hs2 = s2 + i2.D;
             const bool ds2 = hrow && __all(hs2 > hpos) != 0;
             const bool dsd2 = __all(!(hs2 > Num<T>::eps)) != 0;
+            votes |= (ds2 ? 0x10000u : 0u) | (dsd2 ? 0x1000000u : 0u);
             T sq2 = T(0), u2 = T(0), v2 = T(0);
             if (ds2) {
                 sq2 = Num<T>::sqrt(hs2);
@@ -425,12 +430,12 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
                 const bool yac = (g < 2) || (g > ny - 2);
                 MF<T> f;
-                if (ds5 && ds4 && ds3 && ds2 && !yac) {
+                if ((votes & 0xf0000u) == 0xf0000u && !yac) {
                     DC<T> A, B;
                     A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
                     B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
                     f = fast_mom_y(A, B, fg, dt, cd2s);
-                } else if (dsd4 && dsd3 && g - 1 >= 0 && g < ny) {
+                } else if ((votes & 0x6000000u) == 0x6000000u && g - 1 >= 0 && g < ny) {
                     f = kDryMF;  // the face's cells are rows r-4 and r-3 (no clamping here)
                 } else {
                     SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx), A = sc_of(s4, i4.D, i4.qy, i4.qx);
@@ -490,10 +495,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             m2 = m1; m1 = m0;
             X4 = X3; Y4 = Y3; W4 = W3;
             X3 = X2; Y3 = Y2; W3 = W2;
-            din3 = din2; din2 = din1; din1 = din0;
-            ds5 = ds4; ds4 = ds3; ds3 = ds2;
-            dry2 = dry1; dry1 = dry0;
-            dsd4 = dsd3; dsd3 = dsd2;
+            votes = (votes << 1) & 0x060e060eu;  // the bits of rows that stay in the pipeline
         };
 
         // rows y0-4 .. y1+3
