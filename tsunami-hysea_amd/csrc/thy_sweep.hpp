@@ -215,7 +215,7 @@ template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T q
 // start at x0 and whose first row is y0, with bc filled by begin_step for this launch.
 // Returns this lane's minimum of the per-cell dt of the next step (3e38 when it
 // updated no cell, or when x0 lies past the grid).
-template <typename T>
+template <typename T, bool EDGE = true>
 __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCtl<T> &bc, const int x0,
                                              const int y0, const int rows) {
     const int tid = threadIdx.x;
@@ -232,7 +232,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
     const T dt = (T)bc.dt;
     const T tnow = (T)bc.t;
     const int nx = a.nx, ny = a.ny, P = a.pitch;
-    const bool per = a.periodic != 0;
+    const bool per = EDGE && a.periodic != 0;
     const int y1 = min(y0 + rows, ny);
     float dtmin = 3.0e38f;
     bool bad = false;
@@ -240,15 +240,15 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
     if (x0 < nx) {
         const int gx = x0 - SW_HALO + lane;
         // column loaded by this lane: wrapped (periodic) or clamped onto the grid
-        const int cx = per ? ((gx % nx) + nx) % nx : (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx));
+        const int cx = !EDGE ? gx : (per ? ((gx % nx) + nx) % nx : (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)));
         // lanes outside the grid take the e* of the clamped column (clamped stencils)
-        const bool xedge = !per && (x0 - SW_HALO < 0 || x0 + SW_OUT + SW_HALO > nx);
+        const bool xedge = EDGE && !per && (x0 - SW_HALO < 0 || x0 + SW_OUT + SW_HALO > nx);
         const int src = cx - (x0 - SW_HALO);
         // this lane's x face is the left face of its cell: g = gx
-        const int xb = per ? 0 : (gx == 0 ? 1 : (gx == nx ? 2 : 0));
+        const int xb = (!EDGE || per) ? 0 : (gx == 0 ? 1 : (gx == nx ? 2 : 0));
         const int gw = per ? cx : gx;
-        const bool xac = (gw < 2) || (gw > nx - 2);
-        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT && gx < nx;
+        const bool xac = EDGE && ((gw < 2) || (gw > nx - 2));
+        const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT && (!EDGE || gx < nx);
         const bool any_sponge = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
         const int isp = per ? cx : gx;
 
@@ -277,7 +277,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             c.qy = buf_load<T>(rs, vcx, o.qy + so);
             return c;
         };
-        const int vgx = (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)) * (int)sizeof(T);
+        const int vgx = (EDGE ? (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)) : gx) * (int)sizeof(T);
         auto load_metrics = [&](int jn, T &m, T &t) {
             const int so = (jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P * (int)sizeof(T);
             m = buf_load<T>(rs, vgx, o.emax + so);
@@ -286,7 +286,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
 
         // the segment touches the first-order band of the x borders: no x face of it
         // takes the deep form
-        const bool seg_ac = __any(xac) != 0;
+        const bool seg_ac = EDGE && __any(xac) != 0;
         const T cd2s = T(0.5) * (T(1) - a.cvis);
         const T hpos = a.hpos;
 
@@ -612,7 +612,11 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
         if (item >= 0) {
             int x0, y0, n;
             item_origin(a, item, rows, x0, y0, n);
-            if (n > 0) dtmin = sweep_strip(a, bc, x0, y0, n);
+            // segments whose 64 columns lie two columns or more inside the x borders (no
+            // clamped columns, no first-order band, no border faces) take the strip
+            // compiled without the x-edge cases
+            const bool interior = !a.periodic && x0 - SW_HALO >= 2 && x0 + SW_OUT + SW_HALO - 1 <= a.nx - 3;
+            if (n > 0) dtmin = interior ? sweep_strip<T, false>(a, bc, x0, y0, n) : sweep_strip<T, true>(a, bc, x0, y0, n);
         }
     }
     block_dt_min(a, slot, dtmin, s_red);
