@@ -215,6 +215,8 @@ template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T q
 // start at x0 and whose first row is y0, with bc filled by begin_step for this launch.
 // Returns this lane's minimum of the per-cell dt of the next step (3e38 when it
 // updated no cell, or when x0 lies past the grid).
+// EDGE false: the caller guarantees a strip 8 cells or more inside the borders and their
+// sponges, not periodic (no clamping, border faces, first-order band or sponge).
 template <typename T, bool EDGE = true>
 __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCtl<T> &bc, const int x0,
                                              const int y0, const int rows) {
@@ -249,7 +251,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         const int gw = per ? cx : gx;
         const bool xac = EDGE && ((gw < 2) || (gw > nx - 2));
         const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT && (!EDGE || gx < nx);
-        const bool any_sponge = (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
+        const bool any_sponge = EDGE && (a.sp_izq | a.sp_der | a.sp_sup | a.sp_inf) != 0;
         const int isp = per ? cx : gx;
 
         Prm<T> px, py;
@@ -260,8 +262,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         px.bl = a.bizq; px.bh = a.bder;
         py.bl = a.bsup; py.bh = a.binf;
 
-        auto clamp_row = [&](int r) { return r < 0 ? 0 : (r >= ny ? ny - 1 : r); };
-        auto clamp_face = [&](int g) { return g < 0 ? 0 : (g > ny ? ny : g); };
+        auto clamp_row = [&](int r) { return !EDGE ? r : (r < 0 ? 0 : (r >= ny ? ny - 1 : r)); };
+        auto clamp_face = [&](int g) { return !EDGE ? g : (g < 0 ? 0 : (g > ny ? ny : g)); };
         // buffer loads: the row (clamped onto the grid) in the scalar offset, the lane's
         // column in the vector offset; never predicated, so the compiler can keep the
         // prefetches in flight across iterations
@@ -354,8 +356,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             if (r - 1 >= y0 - 2 && r - 1 <= y1 + 2) {
                 const int g = r - 1;
                 const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
-                const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
-                const bool yac = (g < 2) || (g > ny - 2);
+                const int yb = !EDGE ? 0 : (g == 0 ? 1 : (g == ny ? 2 : 0));
+                const bool yac = EDGE && ((g < 2) || (g > ny - 2));
                 T F;
                 if ((votes & 0xfu) == 0xfu && !yac) {
                     F = fast_mass_y(i2.qy, i2.e + i2.D, i1.e + i1.D, fg, dt);
@@ -427,24 +429,26 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             if (r - 3 >= y0 && r - 3 <= y1) {
                 const int g = r - 3;
                 const FaceGeoY<T> fg = uniform_load(a.fy + clamp_face(g));
-                const int yb = g == 0 ? 1 : (g == ny ? 2 : 0);
-                const bool yac = (g < 2) || (g > ny - 2);
+                const int yb = !EDGE ? 0 : (g == 0 ? 1 : (g == ny ? 2 : 0));
+                const bool yac = EDGE && ((g < 2) || (g > ny - 2));
                 MF<T> f;
                 if ((votes & 0xf0000u) == 0xf0000u && !yac) {
                     DC<T> A, B;
                     A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
                     B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
                     f = fast_mom_y(A, B, fg, dt, cd2s);
-                } else if ((votes & 0x6000000u) == 0x6000000u && g - 1 >= 0 && g < ny) {
+                } else if ((votes & 0x6000000u) == 0x6000000u && (!EDGE || (g - 1 >= 0 && g < ny))) {
                     f = kDryMF;  // the face's cells are rows r-4 and r-3 (no clamping here)
                 } else {
                     SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx), A = sc_of(s4, i4.D, i4.qy, i4.qx);
                     SC<T> B = sc_of(s3, i3.D, i3.qy, i3.qx), Q = sc_of(s2, i2.D, i2.qy, i2.qx);
                     // rows outside the grid take the e* of the edge row (clamped stencil)
-                    if (g - 2 < 0) M = (g - 1 < 0) ? B : A;
-                    if (g - 1 < 0) A = B;
-                    if (g >= ny) B = A;
-                    if (g + 1 >= ny) Q = (g >= ny) ? A : B;
+                    if (EDGE) {
+                        if (g - 2 < 0) M = (g - 1 < 0) ? B : A;
+                        if (g - 1 < 0) A = B;
+                        if (g >= ny) B = A;
+                        if (g + 1 >= ny) Q = (g >= ny) ? A : B;
+                    }
                     f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1, fg.valy, dt,
                                           py);
                 }
@@ -612,10 +616,13 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_sweep(const StepArgs<T> a, co
         if (item >= 0) {
             int x0, y0, n;
             item_origin(a, item, rows, x0, y0, n);
-            // segments whose 64 columns lie two columns or more inside the x borders (no
-            // clamped columns, no first-order band, no border faces) take the strip
-            // compiled without the x-edge cases
-            const bool interior = !a.periodic && x0 - SW_HALO >= 2 && x0 + SW_OUT + SW_HALO - 1 <= a.nx - 3;
+            // strips whose rows and 64 columns lie 8 cells or more inside the borders and
+            // their sponges (no clamped rows or columns, no first-order band, no border
+            // faces, no sponge) take the strip compiled without the edge cases
+            constexpr int m = 8;
+            const bool interior = !a.periodic && x0 - SW_HALO - m >= a.sp_izq &&
+                                  x0 + SW_OUT + SW_HALO + m <= a.nx - a.sp_der && y0 - m >= a.sp_sup &&
+                                  y0 + n + m <= a.ny - a.sp_inf;
             if (n > 0) dtmin = interior ? sweep_strip<T, false>(a, bc, x0, y0, n) : sweep_strip<T, true>(a, bc, x0, y0, n);
         }
     }
