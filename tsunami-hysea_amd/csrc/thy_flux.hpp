@@ -22,6 +22,10 @@ template <> struct Num<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // a * b + c rounded once (v_fma_f32).  The library is compiled without automatic
+    // contraction, so every fused product is written out here, at the same place in each
+    // copy of a formula, and all kernels round it the same way.
+    static __device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
     // a / sqrt(x) with one transcendental (reciprocal square root)
     static __device__ __forceinline__ float div_sqrt(float a, float x) { return a * __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float exp(float x) {
@@ -39,6 +43,8 @@ template <> struct Num<double> {
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    // FP64 keeps the product and the sum rounded separately, as the oracle rounds them
+    static __device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
     static __device__ __forceinline__ double div_sqrt(double a, double x) { return a / ::sqrt(x); }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double pow43(double x) { return ::pow(x, 4.0 / 3.0); }
@@ -57,6 +63,10 @@ template <typename T> struct SC {
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+// |a| as a source modifier of the instruction that uses it (no compare and select); where
+// it is added to a positive number the sum equals tabs(a) + that number
+__device__ __forceinline__ float habs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double habs(double a) { return __builtin_fabs(a); }
 template <typename T> __device__ __forceinline__ T sgn(T x) {
     return (x > T(0) ? T(1) : T(0)) - (x < T(0) ? T(1) : T(0));
 }
@@ -65,7 +75,7 @@ template <typename T> __device__ __forceinline__ T sgn(T x) {
 template <typename T> __device__ __forceinline__ T desing(T h, T eps_h) {
     T m = tmax(h, eps_h);
     T h2 = h * h, m2 = m * m;
-    return Num<T>::div_sqrt(T(1.4142135623730951) * h, h2 * h2 + m2 * m2);
+    return Num<T>::div_sqrt(T(1.4142135623730951) * h, Num<T>::mad(h2, h2, m2 * m2));
 }
 
 // smooth monotone limiter: src/GPU/Arista_kernel.cu:45-68
@@ -77,11 +87,11 @@ template <typename T> __device__ __forceinline__ T limiter(T p0m, T p0, T p1, T 
     c1 = tabs(c1);
     if (c0 < Num<T>::eps) return T(1);
     T aux = tabs(Num<T>::div(c1 - c0, c1 + c0));
-    c1 = T(0.5) * (c1 + c0) * (T(1) - aux * aux * aux);
+    c1 = T(0.5) * (c1 + c0) * Num<T>::mad(-(aux * aux), aux, T(1));
     // the smaller over the larger: one division (select first, then divide)
     const bool big0 = c0 > c1;
     T vs = Num<T>::div(big0 ? c1 : c0, (big0 ? c0 : c1) + Num<T>::eps);
-    return Num<T>::div(sg * vs * (T(1) + vs), T(1) + vs * vs);
+    return Num<T>::div(sg * vs * (T(1) + vs), Num<T>::mad(vs, vs, T(1)));
 }
 
 // max(e1 + Dm, 0) - max(e0 + Dm, 0) with Dm = min(D0, D1)  (src/GPU/Arista_kernel.cu:201-202)
@@ -107,7 +117,7 @@ template <typename T> __device__ __forceinline__ T blend(T hij, T hpos, bool alf
 // so a face evaluated by two wavefronts has one value.
 template <typename T> __device__ __forceinline__ T positivity(T F, T h0, T h1, T k0, T k1, T dt) {
     const T lim = (F > T(0)) ? h0 * k0 : h1 * k1;
-    const T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    const T den = habs(F) + Num<T>::eps;
     if (dt * den > lim) {
         const T dt0 = Num<T>::div(lim, den);
         if (dt > dt0) F *= Num<T>::div(dt0, dt + Num<T>::eps);
@@ -163,7 +173,7 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
     q0 = u0 * a.h;
     q1 = u1 * b.h;
     T s0 = Num<T>::sqrt(a.h), s1 = Num<T>::sqrt(b.h);
-    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T uij = Num<T>::div(Num<T>::mad(s0, u0, s1 * u1), s0 + s1 + Num<T>::eps);
     T val = valf * dt;   // 2*len*dt/(A0+A1) with len/cos for x
     T c0 = Num<T>::sqrt(hij);
     T l1 = tmin(uij - c0, u0 - s0);
@@ -176,8 +186,8 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
     }
     T a0;
     if (tabs(l1 - l2) >= Num<T>::eps) {
-        T alr = sgn(l1) * (T(1) - xiL) + val * l1 * xiL;
-        T blr = sgn(l2) * (T(1) - xiR) + val * l2 * xiR;
+        T alr = Num<T>::mad(val * l1, xiL, sgn(l1) * (T(1) - xiL));
+        T blr = Num<T>::mad(val * l2, xiR, sgn(l2) * (T(1) - xiR));
         a0 = Num<T>::div((alr - blr) * (l1 * l2), l2 - l1);
     } else {
         a0 = lmax;
@@ -185,16 +195,16 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
     T F2s, Fw;
     if (YFACE) {
         F2s = len * q0 * g0 * normal;
-        T deta = T(0.5) * (g0 + g1) * deta_of(a.e, b.e, a.D, b.D) - uij * uij * (g1 - g0);
-        Fw = T(0.5) * (q0 * g0 + q1 * g1 - a0 * deta) * len;
+        T deta = Num<T>::mad(-(uij * uij), g1 - g0, T(0.5) * (g0 + g1) * deta_of(a.e, b.e, a.D, b.D));
+        Fw = T(0.5) * Num<T>::mad(-a0, deta, Num<T>::mad(q0, g0, q1 * g1)) * len;
     } else {
         T il = len * g0;
         F2s = il * q0 * normal;
         T deta = deta_of(a.e, b.e, a.D, b.D);
-        Fw = T(0.5) * (q1 + q0 - a0 * deta) * il;
+        Fw = T(0.5) * Num<T>::mad(-a0, deta, q1 + q0) * il;
     }
     T alfa = blend(hij, p.hpos, alfa_cero);
-    T F = alfa * F2s + (T(1) - alfa) * Fw;
+    T F = Num<T>::mad(alfa, F2s, (T(1) - alfa) * Fw);
     return positivity(F, a.h, b.h, k0, k1, dt);
 }
 
@@ -221,8 +231,8 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         T ia = Num<T>::rcp(a.h), ib = Num<T>::rcp(b.h);
         T u0 = a.qn * ia, u1 = b.qn * ib;
         T s0 = Num<T>::sqrt(a.h), s1 = Num<T>::sqrt(b.h);
-        T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
-        T lmax = tabs(uij) + Num<T>::sqrt(hij);
+        T uij = Num<T>::div(Num<T>::mad(s0, u0, s1 * u1), s0 + s1 + Num<T>::eps);
+        T lmax = habs(uij) + Num<T>::sqrt(hij);
         T deta = deta_of(a.e, b.e, a.D, b.D);
         T val = valf * dt;
         bool up = uij < T(0);
@@ -230,15 +240,15 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         T qnn = up ? b.qn : a.qn;
         T qtt = up ? b.qt : a.qt;
         if (YFACE) {
-            deta = T(0.5) * (g0 + g1) * deta - uij * uij * (g1 - g0);
+            deta = Num<T>::mad(-(uij * uij), g1 - g0, T(0.5) * (g0 + g1) * deta);
             T fn = uu * qnn * len;
             T mn, pn;
             if (up) { pn = -fn; mn = fn * (g1 * ic0); }
             else { mn = fn; pn = -fn * g0 * ic1; }
-            mn += hij * deta * len * ic0;
+            mn = Num<T>::mad(hij * deta * len, ic0, mn);
             T DES = p.cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
-            o.mn = mn - DES * ic0;
-            o.pn = pn + DES * ic1;
+            o.mn = Num<T>::mad(-DES, ic0, mn);
+            o.pn = Num<T>::mad(DES, ic1, pn);
             T ft = uu * qtt * len;
             o.mt = ft;
             o.pt = -ft;
@@ -247,7 +257,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
             T il = len * g0;
             T fn = uu * qnn * il;
             T DES = p.cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
-            o.mn = fn + hij * deta * il - DES;
+            o.mn = Num<T>::mad(hij * deta, il, fn) - DES;
             o.pn = -fn + DES;
             T ft = uu * qtt * il;
             o.mt = ft;
@@ -275,7 +285,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     T u0 = f0 * q0, u1 = f1 * q1;
     if (!YFACE) { q0 = u0 * h0; q1 = u1 * h1; }
     T s0 = Num<T>::sqrt(h0), s1 = Num<T>::sqrt(h1);
-    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T uij = Num<T>::div(Num<T>::mad(s0, u0, s1 * u1), s0 + s1 + Num<T>::eps);
     T val = valf * dt;
     T c0 = Num<T>::sqrt(hij);
     T l1 = tmin(uij - c0, u0 - s0);
@@ -284,7 +294,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     T deta = deta_of(a.e, b.e, a.D, b.D);
     T fl0x, fl0y, fl1x, fl1y;
     if (YFACE) {
-        deta = T(0.5) * (g0 + g1) * deta - uij * uij * (g1 - g0);
+        deta = Num<T>::mad(-(uij * uij), g1 - g0, T(0.5) * (g0 + g1) * deta);
         fl1x = q1 * g1; fl1y = u1 * fl1x;
         fl0x = q0 * g0; fl0y = u0 * fl0x;
     } else {
@@ -324,19 +334,19 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     }
     T a0, a1;
     if (tabs(l1 - l2) >= Num<T>::eps) {
-        T alr = sgn(l1) * (T(1) - xiL) + val * l1 * xiL;
-        T blr = sgn(l2) * (T(1) - xiR) + val * l2 * xiR;
+        T alr = Num<T>::mad(val * l1, xiL, sgn(l1) * (T(1) - xiL));
+        T blr = Num<T>::mad(val * l2, xiR, sgn(l2) * (T(1) - xiR));
         T aux = Num<T>::rcp(l2 - l1);
         a0 = (alr - blr) * (l1 * l2) * aux;
-        a1 = (blr * l2 - alr * l1) * aux;
+        a1 = Num<T>::mad(blr, l2, -(alr * l1)) * aux;
     } else {
         a0 = lmax;
         a1 = T(0);
     }
-    T wy = T(0.5) * (fl1y - fl0y + hij * deta);
+    T wy = T(0.5) * Num<T>::mad(hij, deta, fl1y - fl0y);
     T DESx = T(0.5) * a0 * deta;
-    T DESy = T(0.5) * a0 * dq + a1 * wy;
-    T mxw = (T(0.5) * dq + (fl0x - DESx)) * L;
+    T DESy = Num<T>::mad(a1, wy, T(0.5) * a0 * dq);
+    T mxw = Num<T>::mad(T(0.5), dq, fl0x - DESx) * L;
     T myw = (wy + (fl0y - DESy)) * L;
     T pyw = (wy + (DESy - fl1y)) * L;
     // limited upwinding of the tangential velocity
@@ -351,8 +361,9 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     T us = tabs(uij) * sgm;
     T xi = alfa_cero ? T(0) : limiter(p0m, p0, p1, p1p, us);
     T uijt;
-    if (YFACE) uijt = T(0.5) * (p0 * ic0 + p1 * ic1) - T(0.5) * ((T(1) - xi) * sgm + xi * us * val) * (p1 * ic1 - p0 * ic0);
-    else uijt = T(0.5) * (p0 + p1) - T(0.5) * ((T(1) - xi) * sgm + xi * us * val) * (p1 - p0);
+    const T wt = T(-0.5) * Num<T>::mad(xi * us, val, (T(1) - xi) * sgm);
+    if (YFACE) uijt = Num<T>::mad(wt, Num<T>::mad(p1, ic1, -(p0 * ic0)), T(0.5) * Num<T>::mad(p0, ic0, p1 * ic1));
+    else uijt = Num<T>::mad(wt, p1 - p0, T(0.5) * (p0 + p1));
     T aux = mxw * uijt;
     T f3m_n, f3m_t, f3p_n, f3p_t;
     if (YFACE) {
@@ -364,10 +375,10 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     }
     T alfa = blend(hij, p.hpos, alfa_cero);
     T oma = T(1) - alfa;
-    o.mn = alfa * sm_n + oma * f3m_n;
-    o.mt = alfa * sm_t + oma * f3m_t;
-    o.pn = alfa * sp_n + oma * f3p_n;
-    o.pt = alfa * sp_t + oma * f3p_t;
+    o.mn = Num<T>::mad(alfa, sm_n, oma * f3m_n);
+    o.mt = Num<T>::mad(alfa, sm_t, oma * f3m_t);
+    o.pn = Num<T>::mad(alfa, sp_n, oma * f3p_n);
+    o.pt = Num<T>::mad(alfa, sp_t, oma * f3p_t);
     if (lmax < p.eps_h) lmax += p.eps_h;
     o.w = L * lmax;
     return o;
@@ -380,7 +391,7 @@ __device__ __forceinline__ T init_wave(T h0, T q0, T h1, T q1, T eps_h) {
     T u0 = desing(h0, eps_h) * q0;
     T u1 = desing(h1, eps_h) * q1;
     T s0 = Num<T>::sqrt(h0), s1 = Num<T>::sqrt(h1);
-    T uij = Num<T>::div(s0 * u0 + s1 * u1, s0 + s1 + Num<T>::eps);
+    T uij = Num<T>::div(Num<T>::mad(s0, u0, s1 * u1), s0 + s1 + Num<T>::eps);
     T lmax = tabs(uij) + Num<T>::sqrt(hij);
     if (lmax < eps_h) lmax += eps_h;
     return lmax;

@@ -375,7 +375,7 @@ __device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T kA, T dt) {
 #pragma clang fp contract(off)
     T F = q0 * il;
     T lim = (F > T(0)) ? h0 * kA : h1 * kA;
-    T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    T den = habs(F) + Num<T>::eps;
     if (dt * den > lim) {
         T dt0 = lim * Num<T>::rcp(den);
         if (dt > dt0) F *= dt0 * Num<T>::rcp(dt + Num<T>::eps);
@@ -390,7 +390,7 @@ __device__ __forceinline__ T deep_mass_y(T q0, T h0, T h1, T len, T c0, T k0, T 
 #pragma clang fp contract(off)
     T F = len * q0 * c0;
     T lim = (F > T(0)) ? h0 * k0 : h1 * k1;
-    T den = (F > T(0)) ? F + Num<T>::eps : -F + Num<T>::eps;
+    T den = habs(F) + Num<T>::eps;
     if (dt * den > lim) {
         T dt0 = lim * Num<T>::rcp(den);
         if (dt > dt0) F *= dt0 * Num<T>::rcp(dt + Num<T>::eps);
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             const T Ft = deep_mass_y(qy, h, ht, ft.len, ft.cos0, ft.k0, ft.k1, dt);
             const T acc = (Fl - Fr) + (Fb * fb.ic1 - Ft * ft.ic0);
             const T val = dt * rg.invA;
-            T es = e + val * acc;
+            T es = Num<T>::mad(val, acc, e);
             if (!(es + D > T(0))) es = -D;
             const T hs = es + D;
             deep2 = deep2 && (hs > a.hpos);
@@ -553,8 +553,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T h0 = s_es[c0] + f_D[c0], h1 = s_es[c1] + f_D[c1];
                 const T s0 = f_s[c0], s1 = f_s[c1], u0 = f_u[c0], u1 = f_u[c1];
                 const T hij = T(0.5) * (h0 + h1);
-                const T uij = (s0 * u0 + s1 * u1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
-                const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+                const T uij = Num<T>::mad(s0, u0, s1 * u1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
+                const T lmax = habs(uij) + Num<T>::sqrt(hij);
                 const T deta = deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]);
                 const T il = rg.lenx * rg.gx;
                 const bool up = uij < T(0);
@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T fn = uu * (up ? f_qx[c1] : f_qx[c0]) * il;
                 const T ftg = uu * (up ? f_qy[c1] : f_qy[c0]) * il;
                 const T DES = cd2s * (rg.valx * dt) * lmax * lmax * (f_qx[c1] - f_qx[c0]) * il;
-                fx[4 * idx + 0] = fn + hij * deta * il - DES;
+                fx[4 * idx + 0] = Num<T>::mad(hij * deta, il, fn) - DES;
                 fx[4 * idx + 1] = -fn + DES;
                 fx[4 * idx + 2] = ftg;
                 fx[4 * idx + 3] = il * lmax;
@@ -575,20 +575,20 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T h0 = s_es[c0] + f_D[c0], h1 = s_es[c1] + f_D[c1];
                 const T s0 = f_s[c0], s1 = f_s[c1], v0 = f_v[c0], v1 = f_v[c1];
                 const T hij = T(0.5) * (h0 + h1);
-                const T uij = (s0 * v0 + s1 * v1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
-                const T lmax = tabs(uij) + Num<T>::sqrt(hij);
-                const T deta = T(0.5) * (fg.cos0 + fg.cos1) * deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]) -
-                               uij * uij * (fg.cos1 - fg.cos0);
+                const T uij = Num<T>::mad(s0, v0, s1 * v1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
+                const T lmax = habs(uij) + Num<T>::sqrt(hij);
+                const T deta = Num<T>::mad(-(uij * uij), fg.cos1 - fg.cos0,
+                                           T(0.5) * (fg.cos0 + fg.cos1) * deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]));
                 const bool up = uij < T(0);
                 const T uu = up ? v1 : v0;
                 const T fn = uu * (up ? f_qy[c1] : f_qy[c0]) * fg.len;
                 T mn, pn;
                 if (up) { pn = -fn; mn = fn * (fg.cos1 * fg.ic0); }
                 else { mn = fn; pn = -fn * fg.cos0 * fg.ic1; }
-                mn += hij * deta * fg.len * fg.ic0;
+                mn = Num<T>::mad(hij * deta * fg.len, fg.ic0, mn);
                 const T DES = cd2s * (fg.valy * dt) * lmax * lmax * (f_qy[c1] * fg.cos1 - f_qy[c0] * fg.cos0) * fg.len;
-                fyv[4 * idx + 0] = mn - DES * fg.ic0;
-                fyv[4 * idx + 1] = pn + DES * fg.ic1;
+                fyv[4 * idx + 0] = Num<T>::mad(-DES, fg.ic0, mn);
+                fyv[4 * idx + 1] = Num<T>::mad(DES, fg.ic1, pn);
                 fyv[4 * idx + 2] = uu * (up ? f_qx[c1] : f_qx[c0]) * fg.len;
                 fyv[4 * idx + 3] = fg.len * lmax;
             }
@@ -607,8 +607,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T dtc = (aw < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, aw);
                 dtmin = fminf(dtmin, (float)dtc);
                 const T val = dt * rg.invA;
-                const T qx = f_qx[so] + val * ax;
-                const T qy = f_qy[so] + val * ay;
+                const T qx = Num<T>::mad(val, ax, f_qx[so]);
+                const T qy = Num<T>::mad(val, ay, f_qy[so]);
                 const T es = s_es[so];
                 const int64_t go = (int64_t)gy * P + gx;
                 bad = bad || bad_cell(es, qx, qy);
@@ -702,8 +702,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T F = mass_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.k0, fg.k1, fg.valy, dt, py);
                 const bool lo = fj >= 1 && g - 1 >= 0;
                 const bool hi = fj <= SY - 1 && g <= ny - 1;
-                if (lo) s_am[(fj - 1) * SX + si] -= F * fg.ic0;
-                if (hi) s_am[fj * SX + si] += ((border == 1) ? -F : F) * fg.ic1;
+                if (lo) s_am[(fj - 1) * SX + si] = Num<T>::mad(-F, fg.ic0, s_am[(fj - 1) * SX + si]);
+                if (hi) s_am[fj * SX + si] = Num<T>::mad((border == 1) ? -F : F, fg.ic1, s_am[fj * SX + si]);
             }
             __syncthreads();
         }
@@ -718,7 +718,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             if (gy >= 0 && gy < ny && (per || (gx >= 0 && gx < nx))) {
                 const RowGeo<T> rg = a.row[gy];
                 const T D = s_D[io];
-                e = e + dt * rg.invA * s_am[idx];
+                e = Num<T>::mad(dt * rg.invA, s_am[idx], e);
                 if (!(e + D > T(0))) e = -D;
                 if (any_sponge) e = sponge_eta(a, e, D, per ? ((gx % nx) + nx) % nx : gx, gy, rg.dxs, rg.dys);
             }
@@ -849,8 +849,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         const T dtc = (aw < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, aw);
         dtmin = fminf(dtmin, (float)dtc);
         const T val = dt * rg.invA;
-        T qx = qxo + val * s_ax[idx];
-        T qy = qyo + val * s_ay[idx];
+        T qx = Num<T>::mad(val, s_ax[idx], qxo);
+        T qy = Num<T>::mad(val, s_ay[idx], qyo);
         if (any_sponge2) sponge_q(a, gx, gy, qx, qy);
         const T es = s_es[so];
         const T hs = es + D;

@@ -98,8 +98,8 @@ template <typename T>
 __device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T len, T g, T valf, T dt, T cd2s) {
     MF<T> o;
     const T hij = T(0.5) * (a.h + b.h);
-    const T uij = Num<T>::div(a.s * a.un + b.s * b.un, a.s + b.s + Num<T>::eps);
-    const T lmax = tabs(uij) + Num<T>::sqrt(hij);
+    const T uij = Num<T>::div(Num<T>::mad(a.s, a.un, b.s * b.un), a.s + b.s + Num<T>::eps);
+    const T lmax = habs(uij) + Num<T>::sqrt(hij);
     const T deta = deta_of(a.e, b.e, a.D, b.D);
     const T val = valf * dt;
     const bool up = uij < T(0);
@@ -109,7 +109,7 @@ __device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T le
     const T il = len * g;
     const T fn = uu * qnn * il;
     const T DES = cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
-    o.mn = fn + hij * deta * il - DES;
+    o.mn = Num<T>::mad(hij * deta, il, fn) - DES;
     o.pn = -fn + DES;
     const T ft = uu * qtt * il;
     o.mt = ft;
@@ -124,9 +124,9 @@ __device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, cons
     MF<T> o;
     const T g0 = f.cos0, g1 = f.cos1, ic0 = f.ic0, ic1 = f.ic1, len = f.len;
     const T hij = T(0.5) * (a.h + b.h);
-    const T uij = Num<T>::div(a.s * a.un + b.s * b.un, a.s + b.s + Num<T>::eps);
-    const T lmax = tabs(uij) + Num<T>::sqrt(hij);
-    const T deta = f.gh * deta_of(a.e, b.e, a.D, b.D) - uij * uij * f.gd;
+    const T uij = Num<T>::div(Num<T>::mad(a.s, a.un, b.s * b.un), a.s + b.s + Num<T>::eps);
+    const T lmax = habs(uij) + Num<T>::sqrt(hij);
+    const T deta = Num<T>::mad(-(uij * uij), f.gd, f.gh * deta_of(a.e, b.e, a.D, b.D));
     const T val = f.valy * dt;
     const bool up = uij < T(0);
     const T uu = up ? b.un : a.un;
@@ -136,10 +136,10 @@ __device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, cons
     T mn, pn;
     if (up) { pn = -fn; mn = fn * f.g1ic0; }
     else { mn = fn; pn = -fn * g0 * ic1; }
-    mn += hij * deta * len * ic0;
+    mn = Num<T>::mad(hij * deta * len, ic0, mn);
     const T DES = cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
-    o.mn = mn - DES * ic0;
-    o.pn = pn + DES * ic1;
+    o.mn = Num<T>::mad(-DES, ic0, mn);
+    o.pn = Num<T>::mad(DES, ic1, pn);
     const T ft = uu * qtt * len;
     o.mt = ft;
     o.pt = -ft;
@@ -368,15 +368,15 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0)), Q = sc_of(i0.e, i0.D, T(0), T(0));
                     F = mass_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.k0, fg.k1, fg.valy, dt, py);
                 }
-                m2 -= F * fg.ic0;
-                m1 += ((yb == 1) ? -F : F) * fg.ic1;
+                m2 = Num<T>::mad(-F, fg.ic0, m2);
+                m1 = Num<T>::mad((yb == 1) ? -F : F, fg.ic1, m1);
             }
             // ---- h* (as e*) of row r-2, with the sponge ----
             T s2;
             const RowGeo<T> rg2 = uniform_load(a.row + clamp_row(r - 2));
             const bool hrow = r - 2 >= y0 - 2 && r - 2 <= y1 + 1;  // h* of row r-2 is needed
             if (hrow) {
-                T es = i2.e + dt * rg2.invA * m2;
+                T es = Num<T>::mad(dt * rg2.invA, m2, i2.e);
                 if (!(es + i2.D > T(0))) es = -i2.D;
                 if (any_sponge) es = sponge_eta(a, es, i2.D, isp, r - 2, rg2.dxs, rg2.dys);
                 if (xedge) es = __shfl(es, src, 64);
@@ -479,8 +479,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
                     if (out_col) dtmin = fminf(dtmin, (float)dtc);
                     const T val = dt * rg.invA;
-                    qx = i4.qx + val * X4;
-                    qy = i4.qy + val * Y4;
+                    qx = Num<T>::mad(val, X4, i4.qx);
+                    qy = Num<T>::mad(val, Y4, i4.qy);
                     if (any_sponge) sponge_q(a, gx, ju, qx, qy);
                     const T hs = s4 + D;
                     if (hs <= hpos) near_dry(hs, i4.qx, i4.qy, qx, qy, dt, a.mf0, a.vmax, a.eps_h);
@@ -736,15 +736,15 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             if (kS || r >= y0 - 2) {
                 const FaceGeoY<T> fg = uniform_load(a.fy + r);
                 const T F = fast_mass_y(i1.qy, i1.e + i1.D, h0, fg, dt);
-                m1 -= F * fg.ic0;
-                m0 += F * fg.ic1;
+                m1 = Num<T>::mad(-F, fg.ic0, m1);
+                m0 = Num<T>::mad(F, fg.ic1, m0);
             }
             // h* of row r-1 (rows y0-2 .. y1+1)
             T s1 = T(0), hs1 = T(0), sq1 = T(0), u1 = T(0), v1 = T(0);
             const int j1 = r - 1;
             const RowGeo<T> rg1 = uniform_load(a.row + ((kS || j1 >= y0 - 2) ? j1 : y0 - 2));
             if (kS || (j1 >= y0 - 2 && j1 <= y1 + 1)) {
-                T es = i1.e + dt * rg1.invA * m1;
+                T es = Num<T>::mad(dt * rg1.invA, m1, i1.e);
                 if (!(es + i1.D > T(0))) es = -i1.D;
                 s1 = es;
                 hs1 = es + i1.D;
@@ -796,7 +796,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
                 dtmin = on ? fminf(dtmin, (float)dtc) : dtmin;
                 const T val = dt * rg.invA;
-                const T qx = i2.qx + val * X2, qy = i2.qy + val * Y2;
+                const T qx = Num<T>::mad(val, X2, i2.qx), qy = Num<T>::mad(val, Y2, i2.qy);
                 bad = bad || (on && bad_cell(s2, qx, qy));
                 buf_store<T>(s2, rs, voff, o.oe + soff, on);
                 buf_store<T>(qx, rs, voff, o.oqx + soff, on);
