@@ -324,6 +324,8 @@ int grid_1d(int64_t n) {
 template <typename T>
 void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeoY<T>> &fys) {
     const int ny = c->ny;
+    // the 2s numerical-diffusion coefficient as the kernels compute it, in T
+    const T cd2s = T(0.5) * (T(1) - (T)c->prm.cvis);
     rows.resize(ny);
     fys.resize(ny + 1);
     for (int j = 0; j < ny; j++) {
@@ -331,9 +333,10 @@ void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeo
         RowGeo<T> r;
         r.A = (T)A;
         r.invA = (T)(1.0 / A);
-        r.gx = (T)(1.0 / cs);
+        // alto * (1/cos) rounded in T, as the device would multiply the two
+        r.ilx = (T)al * (T)(1.0 / cs);
         r.valx = (T)((2.0 * (al / cs)) / (A + A));
-        r.lenx = (T)al;
+        r.kdx = cd2s * r.valx * r.ilx;
         r.dxs = (T)c->ancho[j];
         r.dys = (T)al;
         r.kA = T(0.25) * r.A;
@@ -357,7 +360,9 @@ void fill_geometry(thy_ctx *c, std::vector<RowGeo<T>> &rows, std::vector<FaceGeo
         f.g1ic0 = (T)(f.cos1 * f.ic0);
         f.k0 = (T)(T(0.25) * f.cos0) * f.A0;
         f.k1 = (T)(T(0.25) * f.cos1) * f.A1;
-        f.pad0 = f.pad1 = f.pad2 = T(0);
+        f.lc0 = (T)(f.len * f.cos0);
+        f.kdy = cd2s * f.valy * f.len;
+        f.pad = T(0);
         fys[g] = f;
     }
 }

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_initial_dt(const StepArgs<T> a, int cur
             T nrm = T(1);
             if (lb) { q0 *= a.bizq; nrm = T(-1); T t = h0; h0 = h1; h1 = t; t = q0; q0 = q1; q1 = t; }
             if (rb) q1 *= a.bder;
-            sum += rg.lenx * rg.gx * init_wave(h0, q0 * nrm, h1, q1 * nrm, a.eps_h);
+            sum += rg.ilx * init_wave(h0, q0 * nrm, h1, q1 * nrm, a.eps_h);
         }
         for (int s = 0; s < 2; s++) {
             const int g = j + s;

@@ -51,7 +51,7 @@ template <> struct Num<double> {
 };
 
 template <typename T> struct Prm {
-    T eps_h, hpos, cvis, cd2s;   // cd2s = 0.5*(1-cvis)
+    T eps_h, hpos, cvis;
     T bl, bh;                    // border factors of the low / high side of this face direction
 };
 
@@ -159,7 +159,7 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
                                        T len, T g0, T g1, T k0, T k1, T valf, T dt, const Prm<T> &p) {
     // deep branch (2s): all four depths above hpos and outside the first-order band
     if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
-        T F = YFACE ? len * a.qn * g0 : a.qn * len * g0;
+        T F = a.qn * (len * g0);
         return positivity(F, a.h, b.h, k0, k1, dt);
     }
     T normal = orient(border, p, m, a, b, q);
@@ -223,7 +223,7 @@ template <typename T> struct MF {
 // ---------------------------------------------------------------------------
 template <typename T, bool YFACE>
 __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int border, bool alfa_cero,
-                                          T len, T g0, T g1, T ic0, T ic1, T valf, T dt, const Prm<T> &p) {
+                                          T len, T g0, T g1, T ic0, T ic1, T valf, T kdes, T dt, const Prm<T> &p) {
     MF<T> o;
     if (!alfa_cero && m.h > p.hpos && a.h > p.hpos && b.h > p.hpos && q.h > p.hpos) {
         // deep-water 2s branch
@@ -234,7 +234,6 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         T uij = Num<T>::div(Num<T>::mad(s0, u0, s1 * u1), s0 + s1 + Num<T>::eps);
         T lmax = habs(uij) + Num<T>::sqrt(hij);
         T deta = deta_of(a.e, b.e, a.D, b.D);
-        T val = valf * dt;
         bool up = uij < T(0);
         T uu = up ? u1 : u0;
         T qnn = up ? b.qn : a.qn;
@@ -246,7 +245,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
             if (up) { pn = -fn; mn = fn * (g1 * ic0); }
             else { mn = fn; pn = -fn * g0 * ic1; }
             mn = Num<T>::mad(hij * deta * len, ic0, mn);
-            T DES = p.cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
+            T DES = kdes * dt * (lmax * lmax) * Num<T>::mad(b.qn, g1, -(a.qn * g0));
             o.mn = Num<T>::mad(-DES, ic0, mn);
             o.pn = Num<T>::mad(DES, ic1, pn);
             T ft = uu * qtt * len;
@@ -256,7 +255,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
         } else {
             T il = len * g0;
             T fn = uu * qnn * il;
-            T DES = p.cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
+            T DES = kdes * dt * (lmax * lmax) * (b.qn - a.qn);
             o.mn = Num<T>::mad(hij * deta, il, fn) - DES;
             o.pn = -fn + DES;
             T ft = uu * qtt * il;

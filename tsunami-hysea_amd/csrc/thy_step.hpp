@@ -61,17 +61,18 @@ template <typename T> struct Arrival {
 };
 
 template <typename T> struct RowGeo {
-    T A, invA, gx, valx;  // area, 1/area, 1/cos(phi), (2*alto/cos)/(2*area)
-    T lenx, dxs, dys, kA;  // alto (x-face length), ancho and alto for sponge distances, A/4
+    T A, invA, ilx, valx;  // area, 1/area, alto*(1/cos(phi)) (x-face length factor), (2*alto/cos)/(2*area)
+    T kdx, dxs, dys, kA;   // 0.5 (1 - cvis) valx ilx (2s diffusion factor), ancho and alto for sponge distances, A/4
 };
 
 template <typename T> struct FaceGeoY {
     T len, cos0, cos1, A0, A1, valy, ic0, ic1; // ancho, cos/area of rows j-1, j (clamped)
-    // face-uniform factors of the deep-water y momentum flux, computed on the host in the
-    // state type exactly as the general faces compute them: 0.5 (cos0 + cos1),
-    // cos1 - cos0, cos1 / cos0 (as cos1 * ic0); and the positivity limiter's factors
-    // cos A / 4 of the two rows
-    T gh, gd, g1ic0, k0, k1, pad0, pad1, pad2;
+    // face-uniform factors of the deep-water y fluxes, computed on the host in the state
+    // type exactly as the general faces compute them: 0.5 (cos0 + cos1), cos1 - cos0,
+    // cos1 / cos0 (as cos1 * ic0), the mass flux factor len * cos0, the 2s diffusion
+    // factor 0.5 (1 - cvis) valy len; and the positivity limiter's factors cos A / 4 of
+    // the two rows
+    T gh, gd, g1ic0, k0, k1, lc0, kdy, pad;
 };
 
 // The dt minimum of a launch is kept as kDtParts partial minima (float bits; for
@@ -388,7 +389,7 @@ __device__ __forceinline__ T deep_mass_x(T q0, T h0, T h1, T il, T kA, T dt) {
 template <typename T>
 __device__ __forceinline__ T deep_mass_y(T q0, T h0, T h1, T len, T c0, T k0, T k1, T dt) {
 #pragma clang fp contract(off)
-    T F = len * q0 * c0;
+    T F = q0 * (len * c0);
     T lim = (F > T(0)) ? h0 * k0 : h1 * k1;
     T den = habs(F) + Num<T>::eps;
     if (dt * den > lim) {
@@ -514,8 +515,8 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             const T h = e + D;
             const T hl = s_e[io - 1] + s_D[io - 1], hr = s_e[io + 1] + s_D[io + 1];
             const T hb = s_e[io - IX] + s_D[io - IX], ht = s_e[io + IX] + s_D[io + IX];
-            const T Fl = deep_mass_x(s_qx[io - 1], hl, h, rg.lenx * rg.gx, rg.kA, dt);
-            const T Fr = deep_mass_x(qx, h, hr, rg.lenx * rg.gx, rg.kA, dt);
+            const T Fl = deep_mass_x(s_qx[io - 1], hl, h, rg.ilx, rg.kA, dt);
+            const T Fr = deep_mass_x(qx, h, hr, rg.ilx, rg.kA, dt);
             const T Fb = deep_mass_y(s_qy[io - IX], hb, h, fb.len, fb.cos0, fb.k0, fb.k1, dt);
             const T Ft = deep_mass_y(qy, h, ht, ft.len, ft.cos0, ft.k0, ft.k1, dt);
             const T acc = (Fl - Fr) + (Fb * fb.ic1 - Ft * ft.ic0);
@@ -544,7 +545,6 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             // per face: mn, pn (normal momentum to the low/high cell), ft (tangential,
             // low -= ft, high += ft), w (wave-speed term)
             T *fx = lds, *fyv = lds + 4 * L::NFX;
-            const T cd2s = T(0.5) * (T(1) - a.cvis);
             for (int idx = tid; idx < L::NFX; idx += NT) {
                 const int tj = idx / (TX + 1), fi = idx - tj * (TX + 1);
                 const int gy = y0 + tj;
@@ -556,12 +556,12 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const T uij = Num<T>::mad(s0, u0, s1 * u1) * Num<T>::rcp(s0 + s1 + Num<T>::eps);
                 const T lmax = habs(uij) + Num<T>::sqrt(hij);
                 const T deta = deta_of(s_es[c0], s_es[c1], f_D[c0], f_D[c1]);
-                const T il = rg.lenx * rg.gx;
+                const T il = rg.ilx;
                 const bool up = uij < T(0);
                 const T uu = up ? u1 : u0;
                 const T fn = uu * (up ? f_qx[c1] : f_qx[c0]) * il;
                 const T ftg = uu * (up ? f_qy[c1] : f_qy[c0]) * il;
-                const T DES = cd2s * (rg.valx * dt) * lmax * lmax * (f_qx[c1] - f_qx[c0]) * il;
+                const T DES = rg.kdx * dt * (lmax * lmax) * (f_qx[c1] - f_qx[c0]);
                 fx[4 * idx + 0] = Num<T>::mad(hij * deta, il, fn) - DES;
                 fx[4 * idx + 1] = -fn + DES;
                 fx[4 * idx + 2] = ftg;
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 if (up) { pn = -fn; mn = fn * (fg.cos1 * fg.ic0); }
                 else { mn = fn; pn = -fn * fg.cos0 * fg.ic1; }
                 mn = Num<T>::mad(hij * deta * fg.len, fg.ic0, mn);
-                const T DES = cd2s * (fg.valy * dt) * lmax * lmax * (f_qy[c1] * fg.cos1 - f_qy[c0] * fg.cos0) * fg.len;
+                const T DES = fg.kdy * dt * (lmax * lmax) * Num<T>::mad(f_qy[c1], fg.cos1, -(f_qy[c0] * fg.cos0));
                 fyv[4 * idx + 0] = Num<T>::mad(-DES, fg.ic0, mn);
                 fyv[4 * idx + 1] = Num<T>::mad(DES, fg.ic1, pn);
                 fyv[4 * idx + 2] = uu * (up ? f_qx[c1] : f_qx[c0]) * fg.len;
@@ -639,7 +639,6 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
     px.eps_h = py.eps_h = a.eps_h;
     px.hpos = py.hpos = a.hpos;
     px.cvis = py.cvis = a.cvis;
-    px.cd2s = py.cd2s = T(0.5) * (T(1) - a.cvis);
     px.bl = a.bizq; px.bh = a.bder;
     py.bl = a.bsup; py.bh = a.binf;
 
@@ -673,7 +672,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
                 const RowGeo<T> rg = a.row[gy];
                 const int lj = sj + 2;
                 SC<T> m = in_cell(fi, lj), c0 = in_cell(fi + 1, lj), c1 = in_cell(fi + 2, lj), q = in_cell(fi + 3, lj);
-                const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, rg.kA, rg.kA, rg.valx, dt, px);
+                const T F = mass_flux<T, false>(m, c0, c1, q, border, ac, rg.ilx, T(1), T(1), rg.kA, rg.kA, rg.valx, dt, px);
                 const bool lo = fi >= 1 && (per || g - 1 >= 0);
                 const bool hi = fi <= SX - 1 && (per || g <= nx - 1);
                 if (lo) s_am[sj * SX + fi - 1] -= F;
@@ -772,7 +771,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             const int sj = tj + 2;
             SC<T> m = st_cell(cxs(fi), sj), c0 = st_cell(cxs(fi + 1), sj);
             SC<T> c1 = st_cell(cxs(fi + 2), sj), q = st_cell(cxs(fi + 3), sj);
-            const MF<T> f = mom_flux<T, false>(m, c0, c1, q, border, ac, rg.lenx, rg.gx, rg.gx, T(1), T(1), rg.valx, dt, px);
+            const MF<T> f = mom_flux<T, false>(m, c0, c1, q, border, ac, rg.ilx, T(1), T(1), T(1), T(1), rg.valx, rg.kdx, dt, px);
             const bool lo = fi >= 1;
             const bool hi = fi <= TX - 1 && (per || g <= nx - 1);
             if (lo) {
@@ -812,7 +811,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
             t0 = c1.qn; c1.qn = c1.qt; c1.qt = t0;
             t0 = q.qn; q.qn = q.qt; q.qt = t0;
             const MF<T> f = mom_flux<T, true>(m, c0, c1, q, border, ac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1,
-                                              fg.valy, dt, py);
+                                              fg.valy, fg.kdy, dt, py);
             const bool lo = fj >= 1;
             const bool hi = fj <= TY - 1 && g <= ny - 1;
             if (lo) {

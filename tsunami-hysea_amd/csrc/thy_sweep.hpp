@@ -73,17 +73,18 @@ template <typename S> __device__ __forceinline__ S uniform_load(const S *p) {
 // roots and velocities computed once per cell (the same arithmetic as the per-lane 2s
 // branch of the general faces, so either path gives a face the same value).
 // ---------------------------------------------------------------------------
-// x face between cells a (left) and b: src/GPU/Arista_kernel.cu:79-95,118-130
+// x face between cells a (left) and b: src/GPU/Arista_kernel.cu:79-95,118-130 (il =
+// alto / cos of the row)
 template <typename T>
-__device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T len, T g, T kA, T dt) {
-    const T F = qa * len * g;
+__device__ __forceinline__ T fast_mass_x(T qa, T ha, T hb, T il, T kA, T dt) {
+    const T F = qa * il;
     return positivity(F, ha, hb, kA, kA, dt);
 }
 
 // y face between rows a (g-1) and b (g): src/GPU/Arista_kernel.cu:97-114,674-686
 template <typename T>
 __device__ __forceinline__ T fast_mass_y(T qa, T ha, T hb, const FaceGeoY<T> &f, T dt) {
-    const T F = f.len * qa * f.cos0;
+    const T F = qa * f.lc0;
     return positivity(F, ha, hb, f.k0, f.k1, dt);
 }
 
@@ -95,20 +96,18 @@ template <typename T> struct DC {
 
 // x face, 2s momentum flux: src/GPU/Arista_kernel.cu:221-280
 template <typename T>
-__device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T len, T g, T valf, T dt, T cd2s) {
+__device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T il, T kd, T dt) {
     MF<T> o;
     const T hij = T(0.5) * (a.h + b.h);
     const T uij = Num<T>::div(Num<T>::mad(a.s, a.un, b.s * b.un), a.s + b.s + Num<T>::eps);
     const T lmax = habs(uij) + Num<T>::sqrt(hij);
     const T deta = deta_of(a.e, b.e, a.D, b.D);
-    const T val = valf * dt;
     const bool up = uij < T(0);
     const T uu = up ? b.un : a.un;
     const T qnn = up ? b.qn : a.qn;
     const T qtt = up ? b.qt : a.qt;
-    const T il = len * g;
     const T fn = uu * qnn * il;
-    const T DES = cd2s * val * lmax * lmax * (b.qn - a.qn) * il;
+    const T DES = kd * dt * (lmax * lmax) * (b.qn - a.qn);
     o.mn = Num<T>::mad(hij * deta, il, fn) - DES;
     o.pn = -fn + DES;
     const T ft = uu * qtt * il;
@@ -120,14 +119,13 @@ __device__ __forceinline__ MF<T> fast_mom_x(const DC<T> &a, const DC<T> &b, T le
 
 // y face, 2s momentum flux: src/GPU/Arista_kernel.cu:780-843
 template <typename T>
-__device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, const FaceGeoY<T> &f, T dt, T cd2s) {
+__device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, const FaceGeoY<T> &f, T dt) {
     MF<T> o;
     const T g0 = f.cos0, g1 = f.cos1, ic0 = f.ic0, ic1 = f.ic1, len = f.len;
     const T hij = T(0.5) * (a.h + b.h);
     const T uij = Num<T>::div(Num<T>::mad(a.s, a.un, b.s * b.un), a.s + b.s + Num<T>::eps);
     const T lmax = habs(uij) + Num<T>::sqrt(hij);
     const T deta = Num<T>::mad(-(uij * uij), f.gd, f.gh * deta_of(a.e, b.e, a.D, b.D));
-    const T val = f.valy * dt;
     const bool up = uij < T(0);
     const T uu = up ? b.un : a.un;
     const T qnn = up ? b.qn : a.qn;
@@ -137,7 +135,7 @@ __device__ __forceinline__ MF<T> fast_mom_y(const DC<T> &a, const DC<T> &b, cons
     if (up) { pn = -fn; mn = fn * f.g1ic0; }
     else { mn = fn; pn = -fn * g0 * ic1; }
     mn = Num<T>::mad(hij * deta * len, ic0, mn);
-    const T DES = cd2s * val * lmax * lmax * (b.qn * g1 - a.qn * g0) * len;
+    const T DES = f.kdy * dt * (lmax * lmax) * Num<T>::mad(b.qn, g1, -(a.qn * g0));
     o.mn = Num<T>::mad(-DES, ic0, mn);
     o.pn = Num<T>::mad(DES, ic1, pn);
     const T ft = uu * qtt * len;
@@ -258,7 +256,6 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         px.eps_h = py.eps_h = a.eps_h;
         px.hpos = py.hpos = a.hpos;
         px.cvis = py.cvis = a.cvis;
-        px.cd2s = py.cd2s = T(0.5) * (T(1) - a.cvis);
         px.bl = a.bizq; px.bh = a.bder;
         py.bl = a.bsup; py.bh = a.binf;
 
@@ -289,7 +286,6 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // the segment touches the first-order band of the x borders: no x face of it
         // takes the deep form
         const bool seg_ac = EDGE && __any(xac) != 0;
-        const T cd2s = T(0.5) * (T(1) - a.cvis);
         const T hpos = a.hpos;
 
         // pipeline registers: input rows r-5..r, e* rows r-5..r-2 (with sqrt(h*) and
@@ -338,7 +334,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
             if (r >= y0 - 2 && r <= y1 + 1) {
                 const RowGeo<T> rg = uniform_load(a.row + clamp_row(r));
                 if (din0 && !seg_ac) {
-                    const T F = fast_mass_x(dpp_left(i0.qx), dpp_left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
+                    const T F = fast_mass_x(dpp_left(i0.qx), dpp_left(h0), h0, rg.ilx, rg.kA, dt);
                     m0 = F - dpp_right(F);
                 } else if (dry0) {
                     m0 = ((xb == 1) ? -kZero : kZero) - kZero;
@@ -348,7 +344,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     const T eR = dpp_right(i0.e), DR = dpp_right(i0.D);
                     const SC<T> M = sc_of(eLL, DLL, T(0), T(0)), A = sc_of(eL, DL, qL, T(0));
                     const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0)), Q = sc_of(eR, DR, T(0), T(0));
-                    const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.lenx, rg.gx, rg.gx, rg.kA, rg.kA, rg.valx, dt, px);
+                    const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.ilx, T(1), T(1), rg.kA, rg.kA, rg.valx, dt, px);
                     m0 = ((xb == 1) ? -F : F) - dpp_right(F);
                 }
             }
@@ -404,7 +400,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 A.h = dpp_left(hs2); A.e = dpp_left(s2); A.D = dpp_left(i2.D);
                 A.qn = dpp_left(i2.qx); A.qt = dpp_left(i2.qy); A.s = dpp_left(sq2); A.un = dpp_left(u2);
                 B.h = hs2; B.e = s2; B.D = i2.D; B.qn = i2.qx; B.qt = i2.qy; B.s = sq2; B.un = u2;
-                const MF<T> f = fast_mom_x(A, B, rg2.lenx, rg2.gx, rg2.valx, dt, cd2s);
+                const MF<T> f = fast_mom_x(A, B, rg2.ilx, rg2.kdx, dt);
                 X2 = -f.pn - dpp_right(f.mn);
                 Y2 = -f.pt - dpp_right(f.mt);
                 W2 = f.w + dpp_right(f.w);
@@ -419,8 +415,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 const T eR = dpp_right(s2), DR = dpp_right(i2.D), qyR = dpp_right(i2.qy);
                 const SC<T> M = sc_of(eLL, DLL, T(0), qyLL), A = sc_of(eL, DL, qxL, qyL);
                 const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy), Q = sc_of(eR, DR, T(0), qyR);
-                const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.lenx, rg2.gx, rg2.gx, T(1), T(1),
-                                                   rg2.valx, dt, px);
+                const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.ilx, T(1), T(1), T(1), T(1),
+                                                   rg2.valx, rg2.kdx, dt, px);
                 X2 = -((xb == 1) ? f.mn : f.pn) - dpp_right(f.mn);
                 Y2 = -((xb == 1) ? f.mt : f.pt) - dpp_right(f.mt);
                 W2 = f.w + dpp_right(f.w);
@@ -436,7 +432,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     DC<T> A, B;
                     A.h = s4 + i4.D; A.e = s4; A.D = i4.D; A.qn = i4.qy; A.qt = i4.qx; A.s = sq4; A.un = v4;
                     B.h = s3 + i3.D; B.e = s3; B.D = i3.D; B.qn = i3.qy; B.qt = i3.qx; B.s = sq3; B.un = v3;
-                    f = fast_mom_y(A, B, fg, dt, cd2s);
+                    f = fast_mom_y(A, B, fg, dt);
                 } else if ((votes & 0x6000000u) == 0x6000000u && (!EDGE || (g - 1 >= 0 && g < ny))) {
                     f = kDryMF;  // the face's cells are rows r-4 and r-3 (no clamping here)
                 } else {
@@ -449,8 +445,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                         if (g >= ny) B = A;
                         if (g + 1 >= ny) Q = (g >= ny) ? A : B;
                     }
-                    f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1, fg.valy, dt,
-                                          py);
+                    f = mom_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.ic0, fg.ic1, fg.valy,
+                                          fg.kdy, dt, py);
                 }
                 Y4 -= f.mn;
                 X4 -= f.mt;
@@ -477,7 +473,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     max_on = on && eo > cmax && eo + D > a.eps_h;
                     arr_on = on && a.track_arrival && !Arrival<T>::arrived(carr) && tabs(eo - carr) > a.dif_at;
                     const T dtc = (W4 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W4);
-                    if (out_col) dtmin = fminf(dtmin, (float)dtc);
+                    dtmin = fminf(dtmin, (float)dtc);  // halo lanes are dropped after the loop
                     const T val = dt * rg.invA;
                     qx = Num<T>::mad(val, X4, i4.qx);
                     qy = Num<T>::mad(val, Y4, i4.qy);
@@ -511,6 +507,7 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // accounts for the previous iteration (see kDropOffset)
         __builtin_amdgcn_s_waitcnt(0);
         for (int r = rbeg; r <= rend; r++) iteration(r, buf, pm, pt);
+        dtmin = out_col ? dtmin : 3.0e38f;
     }
     raise_health(a, bad);
     return dtmin;
@@ -699,7 +696,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
         };
         auto metrics_row = [&](const int j) { return j < y0 ? y0 : (j >= y1 ? y1 - 1 : j); };
         const bool out_col = lane >= SW_HALO && lane < SW_HALO + SW_OUT;
-        const T hpos = a.hpos, cd2s = T(0.5) * (T(1) - a.cvis);
+        const T hpos = a.hpos;
         // rows y0-4 and y1+3 enter only the branch conditions of the outermost mass faces
         valid = __all(ld(o.e, y0 - 4) + ld(o.D, y0 - 4) > hpos) && __all(ld(o.e, y1 + 3) + ld(o.D, y1 + 3) > hpos);
         // pipeline registers: input rows r-2, r-1; h* row r-2 with sqrt(h*), qy/h*;
@@ -729,7 +726,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
             T m0 = T(0);
             if (kS || (r >= y0 - 2 && r <= y1 + 1)) {
                 const RowGeo<T> rg = uniform_load(a.row + r);
-                const T F = fast_mass_x(left(i0.qx), left(h0), h0, rg.lenx, rg.gx, rg.kA, dt);
+                const T F = fast_mass_x(left(i0.qx), left(h0), h0, rg.ilx, rg.kA, dt);
                 m0 = F - right(F);
             }
             // mass y-face g = r between rows r-1 and r (faces y0-2 .. y1+2)
@@ -761,7 +758,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 A.h = left(hs1); A.e = left(s1); A.D = left(i1.D);
                 A.qn = left(i1.qx); A.qt = left(i1.qy); A.s = left(sq1); A.un = left(u1);
                 B.h = hs1; B.e = s1; B.D = i1.D; B.qn = i1.qx; B.qt = i1.qy; B.s = sq1; B.un = u1;
-                const MF<T> f = fast_mom_x(A, B, rg1.lenx, rg1.gx, rg1.valx, dt, cd2s);
+                const MF<T> f = fast_mom_x(A, B, rg1.ilx, rg1.kdx, dt);
                 X1 = -f.pn - right(f.mn);
                 Y1 = -f.pt - right(f.mt);
                 W1 = f.w + right(f.w);
@@ -772,7 +769,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 DC<T> A, B;
                 A.h = s2 + i2.D; A.e = s2; A.D = i2.D; A.qn = i2.qy; A.qt = i2.qx; A.s = sq2; A.un = v2;
                 B.h = hs1; B.e = s1; B.D = i1.D; B.qn = i1.qy; B.qt = i1.qx; B.s = sq1; B.un = v1;
-                const MF<T> f = fast_mom_y(A, B, fg, dt, cd2s);
+                const MF<T> f = fast_mom_y(A, B, fg, dt);
                 Y2 -= f.mn;
                 X2 -= f.mt;
                 W2 += f.w;
@@ -794,7 +791,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
                 buf_store<T>(Arrival<T>::enc(tnow), rs, voff, o.arr + soff,
                               on && a.track_arrival && !Arrival<T>::arrived(carr) && tabs(eo - carr) > a.dif_at);
                 const T dtc = (W2 < Num<T>::eps) ? Num<T>::big : Num<T>::div(T(2) * a.cfl * rg.A, W2);
-                dtmin = on ? fminf(dtmin, (float)dtc) : dtmin;
+                dtmin = in_rows ? fminf(dtmin, (float)dtc) : dtmin;  // halo lanes are dropped after the loop
                 const T val = dt * rg.invA;
                 const T qx = Num<T>::mad(val, X2, i2.qx), qy = Num<T>::mad(val, Y2, i2.qy);
                 bad = bad || (on && bad_cell(s2, qx, qy));
@@ -821,6 +818,7 @@ __device__ __forceinline__ bool deep_strip(const StepArgs<T> &a, const BlockCtl<
 #pragma unroll 1
         for (; r <= y1; r++) row(r, Steady{});
         for (; r <= y1 + 2; r++) row(r, Edge{});
+        dtmin = out_col ? dtmin : 3.0e38f;
     }
     // a strip handed back is redone (and checked) by the general launch
     if (valid) raise_health(a, bad);
