@@ -55,9 +55,11 @@ template <typename T> struct Prm {
     T bl, bh;                    // border factors of the low / high side of this face direction
 };
 
-// One stencil cell: total depth h, eta e, depth D, normal and tangential discharge.
+// One stencil cell: total depth h, eta e, depth D, normal and tangential discharge, and
+// the desingularisation factor desing(h, eps_h) (computed once per cell by the caller,
+// which can share it between the faces of the cell).
 template <typename T> struct SC {
-    T h, e, D, qn, qt;
+    T h, e, D, qn, qt, f;
 };
 
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
@@ -168,8 +170,8 @@ __device__ __forceinline__ T mass_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, int b
     T q0 = a.qn * normal, q1 = b.qn * normal;
     T hij = T(0.5) * (a.h + b.h);
     wet_dry(a, b, p.eps_h, q0, q1);
-    T u0 = desing(a.h, p.eps_h) * q0;
-    T u1 = desing(b.h, p.eps_h) * q1;
+    T u0 = a.f * q0;
+    T u1 = b.f * q1;
     q0 = u0 * a.h;
     q1 = u1 * b.h;
     T s0 = Num<T>::sqrt(a.h), s1 = Num<T>::sqrt(b.h);
@@ -280,7 +282,7 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     T q0 = a.qn * normal, q1 = b.qn * normal;
     T hij = T(0.5) * (h0 + h1);
     wet_dry(a, b, p.eps_h, q0, q1);
-    T f0 = desing(h0, p.eps_h), f1 = desing(h1, p.eps_h);
+    T f0 = a.f, f1 = b.f;
     T u0 = f0 * q0, u1 = f1 * q1;
     if (!YFACE) { q0 = u0 * h0; q1 = u1 * h1; }
     T s0 = Num<T>::sqrt(h0), s1 = Num<T>::sqrt(h1);
@@ -352,10 +354,10 @@ __device__ __forceinline__ MF<T> mom_flux(SC<T> m, SC<T> a, SC<T> b, SC<T> q, in
     T tsign = YFACE ? T(-1) : T(1);
     T q0mt = tsign * m.qt * normal, q0t = tsign * a.qt * normal;
     T q1t = tsign * b.qt * normal, q1pt = tsign * q.qt * normal;
-    T p0m = desing(m.h, p.eps_h) * q0mt;
+    T p0m = m.f * q0mt;
     T p0 = f0 * q0t;
     T p1 = f1 * q1t;
-    T p1p = desing(q.h, p.eps_h) * q1pt;
+    T p1p = q.f * q1pt;
     T sgm = sgn(mxw);
     T us = tabs(uij) * sgm;
     T xi = alfa_cero ? T(0) : limiter(p0m, p0, p1, p1p, us);

@@ -650,6 +650,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         c.h = c.e + c.D;
         c.qn = s_qx[o];
         c.qt = s_qy[o];
+        c.f = desing(c.h, a.eps_h);
         return c;
     };
 
@@ -751,6 +752,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs<T> a, const int slo
         c.h = c.e + c.D;
         c.qn = s_qx[io];
         c.qt = s_qy[io];
+        c.f = desing(c.h, a.eps_h);
         return c;
     };
 

@@ -199,14 +199,21 @@ __device__ __forceinline__ void buf_store<double>(double v, __amdgpu_buffer_rsrc
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, on ? voff : kDropOffset, soff, 0);
 }
 
-template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T qt) {
+// a stencil cell with its desingularisation factor f (shared by the x faces of a lane
+// through lane moves) ...
+template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T qt, T f) {
     SC<T> c;
     c.e = e;
     c.D = D;
     c.h = e + D;
     c.qn = qn;
     c.qt = qt;
+    c.f = f;
     return c;
+}
+// ... or computed for this cell alone (the compiler drops it where no face uses it)
+template <typename T> __device__ __forceinline__ SC<T> sc_of(T e, T D, T qn, T qt, const Prm<T> &p) {
+    return sc_of(e, D, qn, qt, desing(e + D, p.eps_h));
 }
 
 // One wavefront's share of a step: the strip of a.srows rows whose 56 output columns
@@ -342,8 +349,9 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                     const T eL = dpp_left(i0.e), DL = dpp_left(i0.D), qL = dpp_left(i0.qx);
                     const T eLL = dpp_left2(i0.e), DLL = dpp_left2(i0.D);
                     const T eR = dpp_right(i0.e), DR = dpp_right(i0.D);
-                    const SC<T> M = sc_of(eLL, DLL, T(0), T(0)), A = sc_of(eL, DL, qL, T(0));
-                    const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0)), Q = sc_of(eR, DR, T(0), T(0));
+                    const T f0 = desing(h0, px.eps_h);  // the lane's cell, moved to its right neighbour
+                    const SC<T> M = sc_of(eLL, DLL, T(0), T(0), T(0)), A = sc_of(eL, DL, qL, T(0), dpp_left(f0));
+                    const SC<T> B = sc_of(i0.e, i0.D, i0.qx, T(0), f0), Q = sc_of(eR, DR, T(0), T(0), T(0));
                     const T F = mass_flux<T, false>(M, A, B, Q, xb, xac, rg.ilx, T(1), T(1), rg.kA, rg.kA, rg.valx, dt, px);
                     m0 = ((xb == 1) ? -F : F) - dpp_right(F);
                 }
@@ -360,8 +368,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 } else if ((votes & 0x600u) == 0x600u) {
                     F = kZero;
                 } else {
-                    const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0));
-                    const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0)), Q = sc_of(i0.e, i0.D, T(0), T(0));
+                    const SC<T> M = sc_of(i3.e, i3.D, T(0), T(0), T(0)), A = sc_of(i2.e, i2.D, i2.qy, T(0), py);
+                    const SC<T> B = sc_of(i1.e, i1.D, i1.qy, T(0), py), Q = sc_of(i0.e, i0.D, T(0), T(0), T(0));
                     F = mass_flux<T, true>(M, A, B, Q, yb, yac, fg.len, fg.cos0, fg.cos1, fg.k0, fg.k1, fg.valy, dt, py);
                 }
                 m2 = Num<T>::mad(-F, fg.ic0, m2);
@@ -413,8 +421,9 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 const T eL = dpp_left(s2), DL = dpp_left(i2.D), qxL = dpp_left(i2.qx), qyL = dpp_left(i2.qy);
                 const T eLL = dpp_left2(s2), DLL = dpp_left2(i2.D), qyLL = dpp_left2(i2.qy);
                 const T eR = dpp_right(s2), DR = dpp_right(i2.D), qyR = dpp_right(i2.qy);
-                const SC<T> M = sc_of(eLL, DLL, T(0), qyLL), A = sc_of(eL, DL, qxL, qyL);
-                const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy), Q = sc_of(eR, DR, T(0), qyR);
+                const T fs = desing(hs2, px.eps_h);  // the lane's cell, moved to its neighbours
+                const SC<T> M = sc_of(eLL, DLL, T(0), qyLL, dpp_left2(fs)), A = sc_of(eL, DL, qxL, qyL, dpp_left(fs));
+                const SC<T> B = sc_of(s2, i2.D, i2.qx, i2.qy, fs), Q = sc_of(eR, DR, T(0), qyR, dpp_right(fs));
                 const MF<T> f = mom_flux<T, false>(M, A, B, Q, xb, xac, rg2.ilx, T(1), T(1), T(1), T(1),
                                                    rg2.valx, rg2.kdx, dt, px);
                 X2 = -((xb == 1) ? f.mn : f.pn) - dpp_right(f.mn);
@@ -436,8 +445,8 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
                 } else if ((votes & 0x6000000u) == 0x6000000u && (!EDGE || (g - 1 >= 0 && g < ny))) {
                     f = kDryMF;  // the face's cells are rows r-4 and r-3 (no clamping here)
                 } else {
-                    SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx), A = sc_of(s4, i4.D, i4.qy, i4.qx);
-                    SC<T> B = sc_of(s3, i3.D, i3.qy, i3.qx), Q = sc_of(s2, i2.D, i2.qy, i2.qx);
+                    SC<T> M = sc_of(s5, i5.D, i5.qy, i5.qx, py), A = sc_of(s4, i4.D, i4.qy, i4.qx, py);
+                    SC<T> B = sc_of(s3, i3.D, i3.qy, i3.qx, py), Q = sc_of(s2, i2.D, i2.qy, i2.qx, py);
                     // rows outside the grid take the e* of the edge row (clamped stencil)
                     if (EDGE) {
                         if (g - 2 < 0) M = (g - 1 < 0) ? B : A;
