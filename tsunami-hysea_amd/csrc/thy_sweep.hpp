@@ -273,21 +273,26 @@ __device__ __forceinline__ float sweep_strip(const StepArgs<T> &a, const BlockCt
         // prefetches in flight across iterations
         const __amdgpu_buffer_rsrc_t rs = whole_array(a.block, a.block_bytes);
         const StateOffsets<T> o(a, ge, gD, gqx, gqy, oe, oqx, oqy);
+        // the loads carry each array's offset in the vector offset (one register per array,
+        // set once) and only the row in the scalar offset: this loop is short of scalar
+        // registers, not of vector ones
         const int vcx = cx * (int)sizeof(T);
+        const int ve = vcx + o.e, vD = vcx + o.D, vqx = vcx + o.qx, vqy = vcx + o.qy;
         auto load_row = [&](int r) {
             const int so = clamp_row(r) * P * (int)sizeof(T);
             RowCell<T> c;
-            c.e = buf_load<T>(rs, vcx, o.e + so);
-            c.D = buf_load<T>(rs, vcx, o.D + so);
-            c.qx = buf_load<T>(rs, vcx, o.qx + so);
-            c.qy = buf_load<T>(rs, vcx, o.qy + so);
+            c.e = buf_load<T>(rs, ve, so);
+            c.D = buf_load<T>(rs, vD, so);
+            c.qx = buf_load<T>(rs, vqx, so);
+            c.qy = buf_load<T>(rs, vqy, so);
             return c;
         };
         const int vgx = (EDGE ? (gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx)) : gx) * (int)sizeof(T);
+        const int vm = vgx + o.emax, va = vgx + o.arr;
         auto load_metrics = [&](int jn, T &m, T &t) {
             const int so = (jn < y0 ? y0 : (jn >= y1 ? y1 - 1 : jn)) * P * (int)sizeof(T);
-            m = buf_load<T>(rs, vgx, o.emax + so);
-            t = buf_load<T>(rs, vgx, o.arr + so);
+            m = buf_load<T>(rs, vm, so);
+            t = buf_load<T>(rs, va, so);
         };
 
         // the segment touches the first-order band of the x borders: no x face of it
