@@ -90,10 +90,11 @@ template <typename T> __device__ __forceinline__ T limiter(T p0m, T p0, T p1, T 
     if (c0 < Num<T>::eps) return T(1);
     T aux = tabs(Num<T>::div(c1 - c0, c1 + c0));
     c1 = T(0.5) * (c1 + c0) * Num<T>::mad(-(aux * aux), aux, T(1));
-    // the smaller over the larger: one division (select first, then divide)
+    // with r = p / q the smaller over the larger, sg r (1 + r) / (1 + r^2) =
+    // sg p (q + p) / (q^2 + p^2): one division instead of two
     const bool big0 = c0 > c1;
-    T vs = Num<T>::div(big0 ? c1 : c0, (big0 ? c0 : c1) + Num<T>::eps);
-    return Num<T>::div(sg * vs * (T(1) + vs), Num<T>::mad(vs, vs, T(1)));
+    const T pn = big0 ? c1 : c0, qd = (big0 ? c0 : c1) + Num<T>::eps;
+    return Num<T>::div(sg * pn * (qd + pn), Num<T>::mad(qd, qd, pn * pn));
 }
 
 // max(e1 + Dm, 0) - max(e0 + Dm, 0) with Dm = min(D0, D1)  (src/GPU/Arista_kernel.cu:201-202)
